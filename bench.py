@@ -1,0 +1,194 @@
+"""Benchmark of the spline push/pull hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-sweep] [--no-cpu]
+
+Workload ("pushpull_192"): input 4x8x192^3 float32 U[0,1), grid = identity +
+smoothed displacement with 5% of the points out of bounds (seeded,
+nitorch_amd/bench_data.py).  One step = one grid_pull and one grid_push,
+cubic B-splines (order 3), bound dct2, no extrapolation, inputs resident in
+HBM.  `value` = spatial samples processed per second over the step
+(2 x 4 x 192^3 samples per step), in Gvox/s.
+
+Also reported: per-op times, the order x bound sweep of pull/push/count/grad
+(orders 0/1/3/5 x dct2/dft/replicate/zero, one timed launch each after a
+warm-up), the roofline of the dominant kernel (algorithmic bytes per launch /
+average launch time, HIP events on the launch stream), and the reference's
+own C++ CPU backend (oracle/_ref, built from the reference sources) timed on
+a bounded sample on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, published
+METRIC = "pull/push Gvox/s + %HBM peak; GN step solve ms to 1e-4 rel residual"
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--no-sweep", action="store_true")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-sample-batch", type=int, default=1)
+    return p.parse_args()
+
+
+def timed(fn, reps, warmup, stream):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for a, b in ev:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    ms = [a.elapsed_time(b) for a, b in ev]
+    return sum(ms) / len(ms), ms
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
+        sys.exit(2)
+    import torch
+    from nitorch_amd import grid as G
+    from nitorch_amd.bench_data import config2_inputs
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    x_cpu, g_cpu = config2_inputs()
+    x = x_cpu.to(dev)
+    g = g_cpu.to(dev)
+    N, C = x.shape[:2]
+    shape = list(x.shape[2:])
+    S = g[..., 0].numel()           # spatial samples per op
+    inb = ((g >= -0.05) & (g < torch.tensor(shape, device=dev, dtype=g.dtype) - 1 + 0.05)).all(-1)
+    oob_frac = 1.0 - inb.float().mean().item()
+    y = torch.rand_like(x)          # values pushed
+
+    B3, O3 = ["dct2"], [3]
+    pull = lambda: G.grid_pull(x, g, B3, O3, 0)
+    push = lambda: G.grid_push(y, g, shape, B3, O3, 0)
+
+    # timed region: K steps of (pull + push), bracketed by synchronisations
+    for _ in range(args.warmup):
+        pull(); push()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pull(); push()
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+
+    pull_ms, _ = timed(pull, max(args.steps, 5), 1, stream)
+    push_ms, _ = timed(push, max(args.steps, 5), 1, stream)
+
+    # algorithmic bytes per sample (float32): grid 12 + input C*4 + output C*4
+    bytes_per_sample = 12 + 8 * C
+    gb_launch = bytes_per_sample * S / 1e9
+    ops = {
+        "pull_o3_dct2": {"ms": pull_ms, "gvox_s": S / pull_ms / 1e6, "hbm_frac": gb_launch / (pull_ms / 1e3) / HBM_PEAK_GBS},
+        "push_o3_dct2": {"ms": push_ms, "gvox_s": S / push_ms / 1e6, "hbm_frac": gb_launch / (push_ms / 1e3) / HBM_PEAK_GBS},
+    }
+    dom = max(ops, key=lambda k: ops[k]["ms"])
+    achieved = gb_launch / (ops[dom]["ms"] / 1e3)
+
+    sweep = {}
+    if not args.no_sweep:
+        for order in (0, 1, 3, 5):
+            for bound in ("dct2", "dft", "replicate", "zero"):
+                Bb, Oo = [bound], [order]
+                fns = {
+                    "pull": (lambda: G.grid_pull(x, g, Bb, Oo, 0), 12 + 8 * C),
+                    "push": (lambda: G.grid_push(y, g, shape, Bb, Oo, 0), 12 + 8 * C),
+                    "count": (lambda: G.grid_count(g, shape, Bb, Oo, 0), 12 + 4),
+                    "grad": (lambda: G.grid_grad(x, g, Bb, Oo, 0), 12 + 4 * C + 12 * C),
+                }
+                for name, (fn, bps) in fns.items():
+                    ms, _ = timed(fn, 1, 1, stream)
+                    sweep[f"{name}_o{order}_{bound}"] = {
+                        "ms": round(ms, 4), "gvox_s": round(S / ms / 1e6, 2),
+                        "hbm_frac": round(bps * S / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(x_cpu, g_cpu, args.cpu_sample_batch)
+
+    line = {
+        "metric": METRIC,
+        "value": round(2 * S / wall_ms / 1e6, 3),
+        "unit": "Gvox/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded U[0,1) image, smoothed random displacement, 5% out of bounds)",
+        "config": {"workload": "pushpull_192: grid_pull + grid_push, order 3, dct2, "
+                               "input 4x8x192^3, grid 4x192^3x3",
+                   "samples_per_op": S, "oob_fraction": round(oob_frac, 4)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "algorithmic_bytes_per_launch": int(bytes_per_sample * S)},
+        "ops": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in ops.items()},
+        "sweep": sweep,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+def cpu_baseline(x_cpu, g_cpu, nb):
+    """Reference C++ CPU backend (oracle/_ref) on batch[:nb] of the workload
+    (pull and push, order 3, dct2); falls back to the oracle port."""
+    import torch
+    sample = f"batch[:{nb}] of the workload ({nb}x8x192^3 pull + push, order 3, dct2)"
+    x = x_cpu[:nb].contiguous()
+    g = g_cpu[:nb].contiguous()
+    S = g[..., 0].numel()
+    threads = torch.get_num_threads()
+    try:
+        from oracle import ref as R
+        m = R.load()
+    except Exception:
+        m = None
+    if m is not None:
+        B, I = R.bounds(["dct2"]), R.orders([3])
+        t0 = time.perf_counter()
+        m.grid_pull(x, g, B, I, 0)
+        t1 = time.perf_counter()
+        m.grid_push(x, g, list(x.shape[2:]), B, I, 0)
+        t2 = time.perf_counter()
+        kind = "reference"
+    else:
+        from oracle import pushpull as O
+        xs, gs = x[:, :, :48].numpy(), g[:, :48].numpy()
+        S = gs[..., 0].size
+        t0 = time.perf_counter()
+        O.grid_pull(xs, gs, "dct2", 3, 0)
+        t1 = time.perf_counter()
+        O.grid_push(xs, gs, xs.shape[2:], "dct2", 3, 0)
+        t2 = time.perf_counter()
+        kind, threads = "port", 1
+        sample = "first 48 x-slices of batch[:1] (single-threaded oracle port)"
+    return {"value": round(2 * S / (t2 - t0) / 1e9, 4), "unit": "Gvox/s", "cores": threads,
+            "kind": kind, "sample": sample,
+            "pull_s": round(t1 - t0, 3), "push_s": round(t2 - t1, 3)}
+
+
+if __name__ == "__main__":
+    main()

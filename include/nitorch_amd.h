@@ -1,0 +1,142 @@
+/*
+ * nitorch_amd — C ABI of the MI355X (gfx950) spline push/pull and FMG
+ * solver library (libnitorch_amd.so).
+ *
+ * Conventions shared by every entry point
+ *  - all pointers are device pointers (HIP), element strides are int64 and
+ *    counted in elements, never bytes;
+ *  - `dtype` is NI_FLOAT32 or NI_FLOAT64;
+ *  - `bound` and `order` always hold 3 entries, padded by repeating the last
+ *    given value, with the reference's enum values: BoundType replicate=0,
+ *    dct1=1, dct2=2, dst1=3, dst2=4, dft=5, zero=7 (nitorch/_C/src/bounds.h:7-17)
+ *    and InterpolationType 0..7 (nitorch/_C/src/interpolation.h:17-19);
+ *  - tensor layouts are those of the reference's C++ layer: image-like
+ *    tensors [N, C, *spatial] and grids [N, *spatial, dim] with the last dim
+ *    holding (x, y, z) voxel coordinates of the source;
+ *  - outputs are allocated by the caller (with the shapes the reference would
+ *    allocate) and are fully written by the call; push/count outputs must be
+ *    contiguous;
+ *  - `stream` is a hipStream_t (NULL = legacy default stream); every call is
+ *    asynchronous with respect to the host;
+ *  - the return value is 0 on success; on failure it is non-zero and
+ *    ni_last_error() returns a thread-local message (argument errors return
+ *    1, HIP errors return 2).
+ */
+#ifndef NITORCH_AMD_H
+#define NITORCH_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NI_FLOAT32 0
+#define NI_FLOAT64 1
+
+/* Mode flags of ni_pushpull (the reference's do_pull/do_push/do_count/
+ * do_grad/do_sgrad booleans, nitorch/_C/src/impl/pushpull_common.cpp:91-97). */
+#define NI_MODE_PULL 1
+#define NI_MODE_PUSH 2
+#define NI_MODE_COUNT 4
+#define NI_MODE_GRAD 8
+#define NI_MODE_SGRAD 16
+#define NI_MODE_TRGT 32   /* a target tensor is given (push values, grad weights) */
+#define NI_MODE_KTRGT 64  /* target has a trailing dim of size `dim` (grad backward) */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *ni_last_error(void);
+
+/* General entry: one pass of the reference's PushPullImpl with explicit
+ * mode flags (nitorch/_C/src/impl/pushpull_common.cpp:580-794, dispatch
+ * :2494-2609).  src: [N, C, *src_size]; trgt: [N, C, *trgt_size(, dim)];
+ * grid: [N, *trgt_size, dim]; out: pull/sgrad [N, C, *trgt_size(, dim)],
+ * push [N, C, *src_size], count [N, 1, *src_size]; gout: [N, *trgt_size, dim]. */
+int ni_pushpull(int32_t dtype, int32_t dim, const int32_t *bound, const int32_t *order,
+                int32_t extrapolate, int32_t mode, int64_t N, int64_t C,
+                const int64_t *src_size, const int64_t *trgt_size,
+                const void *src, const int64_t *src_stride,
+                const void *trgt, const int64_t *trgt_stride,
+                const void *grid, const int64_t *grid_stride,
+                void *out, const int64_t *out_stride,
+                void *gout, const int64_t *gout_stride, void *stream);
+
+/* Replaces ni::grid_pull (nitorch/_C/src/pushpull.h:10-14, pushpull.cpp:75-107).
+ * input [N, C, *src_size], grid [N, *out_size, dim] -> out [N, C, *out_size]. */
+int ni_grid_pull(int32_t dtype, int32_t dim, const int32_t *bound, const int32_t *order,
+                 int32_t extrapolate, int64_t N, int64_t C, const int64_t *src_size,
+                 const int64_t *out_size, const void *input, const int64_t *input_stride,
+                 const void *grid, const int64_t *grid_stride, void *out,
+                 const int64_t *out_stride, void *stream);
+
+/* Replaces ni::grid_grad (pushpull.h:46-50, pushpull.cpp:268-300).
+ * -> out [N, C, *out_size, dim] (spatial gradients of the spline at the grid). */
+int ni_grid_grad(int32_t dtype, int32_t dim, const int32_t *bound, const int32_t *order,
+                 int32_t extrapolate, int64_t N, int64_t C, const int64_t *src_size,
+                 const int64_t *out_size, const void *input, const int64_t *input_stride,
+                 const void *grid, const int64_t *grid_stride, void *out,
+                 const int64_t *out_stride, void *stream);
+
+/* Replaces ni::grid_push (pushpull.h:22-27, pushpull.cpp:131-184).
+ * input [N, C, *in_size], grid [N, *in_size, dim] -> out [N, C, *out_size]
+ * (contiguous; zero-filled by the call, then scatter-added). */
+int ni_grid_push(int32_t dtype, int32_t dim, const int32_t *bound, const int32_t *order,
+                 int32_t extrapolate, int64_t N, int64_t C, const int64_t *out_size,
+                 const int64_t *in_size, const void *input, const int64_t *input_stride,
+                 const void *grid, const int64_t *grid_stride, void *out,
+                 const int64_t *out_stride, void *stream);
+
+/* Replaces ni::grid_count (pushpull.h:34-38, pushpull.cpp:204-247).
+ * grid [N, *in_size, dim] -> out [N, 1, *out_size] (contiguous). */
+int ni_grid_count(int32_t dtype, int32_t dim, const int32_t *bound, const int32_t *order,
+                  int32_t extrapolate, int64_t N, const int64_t *out_size,
+                  const int64_t *in_size, const void *grid, const int64_t *grid_stride,
+                  void *out, const int64_t *out_stride, void *stream);
+
+/* Replaces ni::grid_pull_backward (pushpull.h:16-20, pushpull.cpp:109-128):
+ * fused push of `grad` (-> grad_input [N, C, *src_size], contiguous) and grid
+ * gradient (-> grad_grid [N, *out_size, dim]).  Pass NULL for an output that
+ * is not needed (the reference's requires_grad tests). */
+int ni_grid_pull_backward(int32_t dtype, int32_t dim, const int32_t *bound, const int32_t *order,
+                          int32_t extrapolate, int64_t N, int64_t C, const int64_t *src_size,
+                          const int64_t *out_size, const void *grad, const int64_t *grad_stride,
+                          const void *input, const int64_t *input_stride, const void *grid,
+                          const int64_t *grid_stride, void *grad_input,
+                          const int64_t *grad_input_stride, void *grad_grid,
+                          const int64_t *grad_grid_stride, void *stream);
+
+/* Replaces ni::grid_push_backward (pushpull.h:28-32, pushpull.cpp:186-202):
+ * grad [N, C, *out_size] (push-output space), input [N, C, *in_size]
+ * -> grad_input = pull(grad) [N, C, *in_size], grad_grid [N, *in_size, dim]. */
+int ni_grid_push_backward(int32_t dtype, int32_t dim, const int32_t *bound, const int32_t *order,
+                          int32_t extrapolate, int64_t N, int64_t C, const int64_t *out_size,
+                          const int64_t *in_size, const void *grad, const int64_t *grad_stride,
+                          const void *input, const int64_t *input_stride, const void *grid,
+                          const int64_t *grid_stride, void *grad_input,
+                          const int64_t *grad_input_stride, void *grad_grid,
+                          const int64_t *grad_grid_stride, void *stream);
+
+/* Replaces ni::grid_count_backward (pushpull.h:40-44, pushpull.cpp:249-265):
+ * grad [N, 1, *out_size] -> grad_grid [N, *in_size, dim]. */
+int ni_grid_count_backward(int32_t dtype, int32_t dim, const int32_t *bound, const int32_t *order,
+                           int32_t extrapolate, int64_t N, const int64_t *out_size,
+                           const int64_t *in_size, const void *grad, const int64_t *grad_stride,
+                           const void *grid, const int64_t *grid_stride, void *grad_grid,
+                           const int64_t *grad_grid_stride, void *stream);
+
+/* Replaces ni::grid_grad_backward (pushpull.h:52-56, pushpull.cpp:303-327):
+ * grad [N, C, *out_size, dim] -> grad_input [N, C, *src_size] (contiguous),
+ * grad_grid [N, *out_size, dim]. */
+int ni_grid_grad_backward(int32_t dtype, int32_t dim, const int32_t *bound, const int32_t *order,
+                          int32_t extrapolate, int64_t N, int64_t C, const int64_t *src_size,
+                          const int64_t *out_size, const void *grad, const int64_t *grad_stride,
+                          const void *input, const int64_t *input_stride, const void *grid,
+                          const int64_t *grid_stride, void *grad_input,
+                          const int64_t *grad_input_stride, void *grad_grid,
+                          const int64_t *grad_grid_stride, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NITORCH_AMD_H */

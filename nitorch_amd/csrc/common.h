@@ -1,0 +1,193 @@
+// Shared device/host helpers for the nitorch_amd HIP library (gfx950).
+//
+//  * boundary folding (index + sign) for the reference's BoundType codes
+//    (nitorch/_C/src/bounds.h:7-17, semantics of bounds_common.h:28-219);
+//  * B-spline weight / derivative evaluation from the generated piece tables
+//    (semantics of interpolation_common.h:57-969);
+//  * error reporting for the C ABI (thread-local message, integer status).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cmath>
+
+#include "spline_tables.h"
+
+namespace ni_amd {
+
+enum BoundCode : int {
+  B_REPLICATE = 0, B_DCT1 = 1, B_DCT2 = 2, B_DST1 = 3, B_DST2 = 4,
+  B_DFT = 5, B_SLIDING = 6, B_ZERO = 7, B_NOCHECK = 8
+};
+
+// --------------------------------------------------------------------------
+// Boundary folding.  Returns the in-range index of `i` on an axis of length
+// `n` and sets `s` to the multiplicative sign (+1, -1, or 0 = "outside, reads
+// as zero").  In-range indices short-circuit (every bound maps them to
+// themselves with sign +1), so the integer modulo only runs on border taps.
+// --------------------------------------------------------------------------
+template <typename I>
+__host__ __device__ inline I pos_mod(I a, I m) {
+  I r = a % m;
+  return r < 0 ? r + m : r;
+}
+
+template <typename I>
+__host__ __device__ inline I fold(int b, I i, I n, int &s) {
+  s = 1;
+  if (i >= 0 && i < n) return i;
+  switch (b) {
+    case B_REPLICATE:
+      return i < 0 ? I(0) : n - 1;
+    case B_DCT1: {
+      if (n == 1) return 0;
+      I p = 2 * (n - 1);
+      I k = pos_mod<I>(i < 0 ? -i : i, p);
+      return k >= n ? p - k : k;
+    }
+    case B_DCT2: {
+      I p = 2 * n;
+      I k = pos_mod<I>(i, p);
+      return k >= n ? p - 1 - k : k;
+    }
+    case B_DST2: {
+      I p = 2 * n;
+      I k = pos_mod<I>(i, p);
+      I j = i < 0 ? n - i - 1 : i;
+      s = ((j / n) & 1) ? -1 : 1;
+      return k >= n ? p - 1 - k : k;
+    }
+    case B_DST1: {
+      if (n == 1) return 0;
+      I p = 2 * (n + 1);
+      I j = i < 0 ? n - i - 1 : i;
+      j = j % p;
+      if (j % (n + 1) == n) s = 0;
+      else s = ((j / (n + 1)) & 1) ? -1 : 1;
+      I k = (i == -1) ? I(0) : (i < 0 ? -i - 2 : i);
+      k = pos_mod<I>(k, p);
+      if (k == n) return n - 1;
+      return k > n ? p - k - 2 : k;
+    }
+    case B_DFT:
+      return pos_mod<I>(i, n);
+    default:  // zero / nocheck / sliding: outside reads as zero
+      s = 0;
+      return 0;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Spline evaluation.  For order O, the taps of a sample at coordinate x are
+// lo, lo+1, ..., lo+O with lo = floor(x - (O-1)/2) (round(x) for O = 0);
+// tap k always falls on the same polynomial piece, so the piece index is a
+// compile-time constant when O is.
+// --------------------------------------------------------------------------
+template <int O>
+__host__ __device__ constexpr int tap_piece(int k) {
+  return (O & 1) ? (k <= (O - 1) / 2 ? (O - 1) / 2 - k : k - (O + 1) / 2)
+                 : (k <= O / 2 ? O / 2 - k : k - O / 2);
+}
+
+template <int O>
+__host__ __device__ constexpr double piece_centre(int p) {
+  return (O & 1) ? p + 0.5 : double(p);
+}
+
+template <typename T, int ND>
+__host__ __device__ inline T horner(const double (&c)[ND], T u) {
+  T r = T(c[ND - 1]);
+#pragma unroll
+  for (int i = ND - 2; i >= 0; --i) r = r * u + T(c[i]);
+  return r;
+}
+
+// weight / derivative / second derivative of tap k at signed distance d
+template <typename T, int O>
+__host__ __device__ inline T spl_w(int k, T d) {
+  if constexpr (O == 0) {
+    return T(1);
+  } else {
+    const int p = tap_piece<O>(k);
+    T u = (d < T(0) ? -d : d) - T(piece_centre<O>(p));
+    return horner<T, SplTab<O>::NDW>(SplTab<O>::W[p], u);
+  }
+}
+
+template <typename T, int O>
+__host__ __device__ inline T spl_g(int k, T d) {
+  if constexpr (O == 0) {
+    return T(0);
+  } else {
+    const int p = tap_piece<O>(k);
+    T a = d < T(0) ? -d : d;
+    T g = horner<T, SplTab<O>::NDG>(SplTab<O>::G[p], a - T(piece_centre<O>(p)));
+    return d < T(0) ? -g : g;
+  }
+}
+
+template <typename T, int O>
+__host__ __device__ inline T spl_h(int k, T d) {
+  if constexpr (O <= 1) {
+    return T(0);
+  } else if constexpr (O == 2) {
+    T a = d < T(0) ? -d : d;
+    return a < T(0.5) ? T(-2) : T(1);  // discontinuous: outer piece at |d| = 1/2
+  } else {
+    const int p = tap_piece<O>(k);
+    T a = d < T(0) ? -d : d;
+    return horner<T, SplTab<O>::NDH>(SplTab<O>::H[p], a - T(piece_centre<O>(p)));
+  }
+}
+
+// Runtime-order variants (anisotropic orders, generic kernel).
+template <typename T, int O>
+__host__ __device__ inline void spl_all_rt(int k, T d, T &w, T &g, T &h) {
+  w = spl_w<T, O>(k, d);
+  g = spl_g<T, O>(k, d);
+  h = spl_h<T, O>(k, d);
+}
+
+template <typename T>
+__host__ __device__ inline void spl_eval(int order, int k, T d, T &w, T &g, T &h) {
+  switch (order) {
+    case 0: spl_all_rt<T, 0>(k, d, w, g, h); break;
+    case 1: spl_all_rt<T, 1>(k, d, w, g, h); break;
+    case 2: spl_all_rt<T, 2>(k, d, w, g, h); break;
+    case 3: spl_all_rt<T, 3>(k, d, w, g, h); break;
+    case 4: spl_all_rt<T, 4>(k, d, w, g, h); break;
+    case 5: spl_all_rt<T, 5>(k, d, w, g, h); break;
+    case 6: spl_all_rt<T, 6>(k, d, w, g, h); break;
+    default: spl_all_rt<T, 7>(k, d, w, g, h); break;
+  }
+}
+
+template <typename T>
+__host__ __device__ inline T first_tap_offset(int order) {
+  return T(order - 1) * T(0.5);
+}
+
+// --------------------------------------------------------------------------
+// Error reporting for the C ABI.
+// --------------------------------------------------------------------------
+void set_error(const char *fmt, ...);
+const char *get_error();
+
+}  // namespace ni_amd
+
+#define NI_CHECK(cond, ...)                     \
+  do {                                          \
+    if (!(cond)) {                              \
+      ::ni_amd::set_error(__VA_ARGS__);         \
+      return 1;                                 \
+    }                                           \
+  } while (0)
+
+#define NI_HIP_CHECK(expr)                                               \
+  do {                                                                   \
+    hipError_t _e = (expr);                                              \
+    if (_e != hipSuccess) {                                              \
+      ::ni_amd::set_error("%s: HIP error %s", __func__,                  \
+                          hipGetErrorString(_e));                        \
+      return 2;                                                          \
+    }                                                                    \
+  } while (0)
