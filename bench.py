@@ -36,7 +36,9 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-sweep", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--cpu-sample-batch", type=int, default=1)
+    p.add_argument("--cpu-slices", type=int, default=48,
+                   help="CPU baseline sample: first K x-slices of batch element 0")
+    p.add_argument("--no-solve", action="store_true")
     return p.parse_args()
 
 
@@ -121,9 +123,13 @@ def main():
                         "ms": round(ms, 4), "gvox_s": round(S / ms / 1e6, 2),
                         "hbm_frac": round(bps * S / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
+    solve = None
+    if not args.no_solve:
+        solve = gn_solve(dev)
+
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(x_cpu, g_cpu, args.cpu_sample_batch)
+        cpu = cpu_baseline(x_cpu, g_cpu, args.cpu_slices)
 
     line = {
         "metric": METRIC,
@@ -147,20 +153,46 @@ def main():
                      "algorithmic_bytes_per_launch": int(bytes_per_sample * S)},
         "ops": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in ops.items()},
         "sweep": sweep,
+        "gn_solve": solve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
 
 
-def cpu_baseline(x_cpu, g_cpu, nb):
-    """Reference C++ CPU backend (oracle/_ref) on batch[:nb] of the workload
-    (pull and push, order 3, dct2); falls back to the oracle port."""
+def gn_solve(dev):
+    """Gauss-Newton step of the fmg_160 workload (160x192x160x3, bending 0.2,
+    membrane 1e-3, absolute 1e-4, lame (0.05, 0.2), dft): plain GPU FMG
+    (2 V-cycles per level, 4 Gauss-Seidel sweeps) then V-cycles until the
+    relative residual is <= 1e-4.  Times the whole solve, after one warm-up."""
     import torch
-    sample = f"batch[:{nb}] of the workload ({nb}x8x192^3 pull + push, order 3, dct2)"
-    x = x_cpu[:nb].contiguous()
-    g = g_cpu[:nb].contiguous()
+    from nitorch_amd.bench_data import config4_inputs
+    from nitorch_amd import solve as SV
+    h, g = config4_inputs()
+    h = h.to(dev).movedim(-1, 1).contiguous()
+    g = g.to(dev).movedim(-1, 1).contiguous()
+    pen = (1e-4, 1e-3, 0.2, 0.05, 0.2)
+    ws = None
+    SV.solve_to_tolerance(h, g, pen, tol=1e-4, nb_cycles=2, nb_iter=4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, rel, ncyc = SV.solve_to_tolerance(h, g, pen, tol=1e-4, nb_cycles=2, nb_iter=4)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3
+    return {"workload": "fmg_160: 1x3x160x192x160, H 6 ch, dft", "ms_to_1e-4": round(ms, 3),
+            "rel_residual": rel, "extra_vcycles": ncyc, "nb_cycles": 2, "nb_iter": 4,
+            "method": "plain GPU FMG (multi-colour Gauss-Seidel)"}
+
+
+def cpu_baseline(x_cpu, g_cpu, slices):
+    """Reference C++ CPU backend (oracle/_ref) on the first `slices` x-slices of
+    batch element 0 (pull and push, order 3, dct2); falls back to the oracle
+    port (single thread) when the reference build is absent."""
+    import torch
+    x = x_cpu[:1].contiguous()
+    g = g_cpu[:1, :slices].contiguous()
     S = g[..., 0].numel()
     threads = torch.get_num_threads()
+    sample = f"batch 0, first {slices} of 192 x-slices of the grid ({S} samples), 8 channels, order 3, dct2"
     try:
         from oracle import ref as R
         m = R.load()
@@ -168,24 +200,26 @@ def cpu_baseline(x_cpu, g_cpu, nb):
         m = None
     if m is not None:
         B, I = R.bounds(["dct2"]), R.orders([3])
+        y = torch.rand((1, x.shape[1]) + tuple(g.shape[1:-1]))
         t0 = time.perf_counter()
         m.grid_pull(x, g, B, I, 0)
         t1 = time.perf_counter()
-        m.grid_push(x, g, list(x.shape[2:]), B, I, 0)
+        m.grid_push(y, g, list(x.shape[2:]), B, I, 0)
         t2 = time.perf_counter()
         kind = "reference"
     else:
         from oracle import pushpull as O
-        xs, gs = x[:, :, :48].numpy(), g[:, :48].numpy()
-        S = gs[..., 0].size
+        g = g[:, :4]
+        S = g[..., 0].numel()
+        y = torch.rand((1, x.shape[1]) + tuple(g.shape[1:-1])).numpy()
         t0 = time.perf_counter()
-        O.grid_pull(xs, gs, "dct2", 3, 0)
+        O.grid_pull(x.numpy(), g.numpy(), "dct2", 3, 0)
         t1 = time.perf_counter()
-        O.grid_push(xs, gs, xs.shape[2:], "dct2", 3, 0)
+        O.grid_push(y, g.numpy(), x.shape[2:], "dct2", 3, 0)
         t2 = time.perf_counter()
         kind, threads = "port", 1
-        sample = "first 48 x-slices of batch[:1] (single-threaded oracle port)"
-    return {"value": round(2 * S / (t2 - t0) / 1e9, 4), "unit": "Gvox/s", "cores": threads,
+        sample = f"batch 0, first 4 x-slices ({S} samples), single-threaded oracle port"
+    return {"value": round(2 * S / (t2 - t0) / 1e9, 5), "unit": "Gvox/s", "cores": threads,
             "kind": kind, "sample": sample,
             "pull_s": round(t1 - t0, 3), "push_s": round(t2 - t1, 3)}
 

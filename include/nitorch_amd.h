@@ -135,6 +135,64 @@ int ni_grid_grad_backward(int32_t dtype, int32_t dim, const int32_t *bound, cons
                           const int64_t *grad_input_stride, void *grad_grid,
                           const int64_t *grad_grid_stride, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Displacement-field solver (3-D).  Fields are [N, 3, *shape] and Hessians
+ * [N, CC, *shape], contiguous, with CC in {1, 3, 6} (6 = h00 h11 h22 h01 h02
+ * h12) or hessian == NULL.  `penalty` = {absolute, membrane, bending,
+ * lame_shear, lame_div}; `voxel_size` has 3 entries.
+ * ---------------------------------------------------------------------- */
+
+/* Replaces ni::regulariser_grid (nitorch/_C/src/regulariser_grid.h:8-11):
+ * output = (H + L) input. */
+int ni_regulariser_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *input, void *output,
+                        const void *hessian, int32_t CC, const int32_t *bound, const double *voxel_size,
+                        const double *penalty, void *stream);
+
+/* Residual g - (H + L) x (the reference's residuals_grid,
+ * nitorch/_C/src/impl/fmg_common.cpp:334-352).  If `norms` (device, 2 doubles)
+ * is given it receives ||g - (H+L)x||^2 and ||g||^2. */
+int ni_residual_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *solution, const void *gradient,
+                     void *residual, const void *hessian, int32_t CC, const int32_t *bound,
+                     const double *voxel_size, const double *penalty, double *norms, void *stream);
+
+/* Replaces ni::relax_grid (nitorch/_C/src/relax_grid.h:8-13): nb_iter
+ * multi-colour Gauss-Seidel sweeps on (H + L) x = g, x updated in place. */
+int ni_relax_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
+                  const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
+                  const double *penalty, int32_t nb_iter, void *stream);
+
+/* Replace ni::fmg_restriction / ni::fmg_prolongation (nitorch/_C/src/multires.h):
+ * [N, C, *in_shape] -> [N, C, *out_shape] (linear-adjoint restriction,
+ * quadratic prolongation). */
+int ni_fmg_restriction(int32_t dtype, int64_t N, int64_t C, const int64_t *in_shape, const int64_t *out_shape,
+                       const void *input, void *output, const int32_t *bound, void *stream);
+int ni_fmg_prolongation(int32_t dtype, int64_t N, int64_t C, const int64_t *in_shape, const int64_t *out_shape,
+                        const void *input, void *output, const int32_t *bound, void *stream);
+
+/* Bytes of device workspace ni_fmg_grid needs for this problem. */
+int64_t ni_fmg_grid_workspace(int32_t dtype, int64_t N, const int64_t *shape, int32_t CC, int32_t max_levels);
+
+/* Replaces ni::fmg_grid (nitorch/_C/src/fmg.h:22-37, solver='relax'):
+ * full multigrid with nb_cycles V-cycles per level and nb_iter relaxation
+ * sweeps, then `extra_vcycles` further V-cycles on the finest level.
+ * `solution` is the initial guess (zeros for none) and receives the result. */
+int ni_fmg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
+                const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
+                const double *penalty, int32_t nb_cycles, int32_t nb_iter, int32_t max_levels,
+                int32_t extra_vcycles, void *workspace, int64_t workspace_bytes, void *stream);
+
+/* Time-to-tolerance solve (no reference counterpart: the benchmark's "GN step
+ * to 1e-4 relative residual"): ni_fmg_grid's full multigrid, then V-cycles on
+ * the finest level until ||g - (H+L)x|| / ||g|| <= tol or max_vcycles.
+ * `norms` is a device buffer of 2 doubles; the call synchronises `stream`
+ * once per V-cycle to read the residual.  Reports the final relative
+ * residual and the number of extra V-cycles. */
+int ni_fmg_solve(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
+                 const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
+                 const double *penalty, int32_t nb_cycles, int32_t nb_iter, int32_t max_levels,
+                 double tol, int32_t max_vcycles, double *norms, void *workspace, int64_t workspace_bytes,
+                 double *out_rel, int32_t *out_vcycles, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

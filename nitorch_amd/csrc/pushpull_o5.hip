@@ -1,0 +1,7 @@
+// Order-5 specialisations of the 3-D float push/pull kernels (one
+// translation unit per order so that the library builds in parallel).
+#include "pushpull_impl.h"
+
+namespace ni_amd {
+NI_DEFINE_DISPATCH_ORDER(5)
+}  // namespace ni_amd

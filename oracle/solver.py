@@ -1,0 +1,129 @@
+"""ORACLE — test infrastructure only.
+
+numpy front-end of `solver_oracle.c` (FP64) for 3-D displacement fields in
+the reference's C++ layout [N, 3, X, Y, Z]; Hessians [N, CC, X, Y, Z] with
+CC in {1, 3, 6} (6 = h00 h11 h22 h01 h02 h12).
+"""
+import ctypes
+
+import numpy as np
+
+from .pushpull import lib as _lib, bound_codes
+
+_c = ctypes
+
+
+def _setup():
+    L = _lib()
+    if getattr(L, "_solver_ready", False):
+        return L
+    D, I64, I = _c.POINTER(_c.c_double), _c.c_int64, _c.c_int
+    PI, PI64 = _c.POINTER(_c.c_int), _c.POINTER(_c.c_int64)
+    L.or_regulariser_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5
+    L.or_regulariser_grid.restype = None
+    L.or_relax_grid.argtypes = [D, D, I, D, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5 + [I, I]
+    L.or_relax_grid.restype = None
+    L.or_prolong.argtypes = [D, D, I64, I64, PI64, PI64, PI]
+    L.or_prolong.restype = None
+    L.or_restrict.argtypes = [D, D, I64, I64, PI64, PI64, PI]
+    L.or_restrict.restype = None
+    L.or_fmg_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5 + [I, I, I, I, I]
+    L.or_fmg_grid.restype = None
+    L.or_cg_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5 + [I, _c.c_double]
+    L.or_cg_grid.restype = I
+    L._solver_ready = True
+    return L
+
+
+def _d(a):
+    return a.ctypes.data_as(_c.POINTER(_c.c_double)) if a is not None else None
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _vs(voxel_size):
+    v = list(np.atleast_1d(voxel_size).astype(float))
+    v = (v + [v[-1]] * 3)[:3]
+    return (_c.c_double * 3)(*v)
+
+
+def _b(bound):
+    return (_c.c_int * 3)(*bound_codes(bound))
+
+
+def _pen(absolute, membrane, bending, lame):
+    lame = list(np.atleast_1d(lame).astype(float)) if lame is not None else [0.0]
+    lame = (lame + [lame[-1]])[:2]
+    return [float(absolute), float(membrane), float(bending), lame[0], lame[1]]
+
+
+def regulariser_grid(x, hessian=None, absolute=0, membrane=0, bending=0, lame=0,
+                     voxel_size=1, bound="dft"):
+    """(H + L) x for x [N, 3, X, Y, Z]."""
+    L = _setup()
+    x, h = _f64(x), _f64(hessian)
+    out = np.empty_like(x)
+    N, _, X, Y, Z = x.shape
+    CC = 0 if h is None else h.shape[1]
+    L.or_regulariser_grid(_d(x), _d(out), _d(h), CC, N, X, Y, Z, _b(bound), _vs(voxel_size),
+                          *_pen(absolute, membrane, bending, lame))
+    return out
+
+
+def relax_grid(g, hessian=None, x=None, absolute=0, membrane=0, bending=0, lame=0,
+               voxel_size=1, bound="dft", nb_iter=1, quirk_fold=False):
+    L = _setup()
+    g, h = _f64(g), _f64(hessian)
+    x = np.zeros_like(g) if x is None else _f64(x).copy()
+    N, _, X, Y, Z = g.shape
+    CC = 0 if h is None else h.shape[1]
+    L.or_relax_grid(_d(g), _d(h), CC, _d(x), N, X, Y, Z, _b(bound), _vs(voxel_size),
+                    *_pen(absolute, membrane, bending, lame), int(nb_iter), int(quirk_fold))
+    return x
+
+
+def prolong(x, out_shape, bound="dct2"):
+    L = _setup()
+    x = _f64(x)
+    N, C = x.shape[:2]
+    out = np.empty((N, C) + tuple(out_shape))
+    L.or_prolong(_d(x), _d(out), N, C, (_c.c_int64 * 3)(*x.shape[2:]), (_c.c_int64 * 3)(*out_shape), _b(bound))
+    return out
+
+
+def restrict(x, out_shape=None, bound="dct2"):
+    L = _setup()
+    x = _f64(x)
+    N, C = x.shape[:2]
+    if out_shape is None:
+        out_shape = tuple(1 + (s - 1) // 2 for s in x.shape[2:])
+    out = np.empty((N, C) + tuple(out_shape))
+    L.or_restrict(_d(x), _d(out), N, C, (_c.c_int64 * 3)(*x.shape[2:]), (_c.c_int64 * 3)(*out_shape), _b(bound))
+    return out
+
+
+def fmg_grid(hessian, g, x=None, absolute=0, membrane=0, bending=0, lame=0, voxel_size=1,
+             bound="dft", nb_cycles=2, nb_iter=2, max_levels=16, quirk_fold=False, quirk_zscale=False):
+    L = _setup()
+    g, h = _f64(g), _f64(hessian)
+    x = np.zeros_like(g) if x is None else _f64(x).copy()
+    N, _, X, Y, Z = g.shape
+    CC = 0 if h is None else h.shape[1]
+    L.or_fmg_grid(_d(h), _d(g), _d(x), CC, N, X, Y, Z, _b(bound), _vs(voxel_size),
+                  *_pen(absolute, membrane, bending, lame), int(nb_cycles), int(nb_iter),
+                  int(max_levels), int(quirk_fold), int(quirk_zscale))
+    return x
+
+
+def cg_grid(hessian, g, x=None, absolute=0, membrane=0, bending=0, lame=0, voxel_size=1,
+            bound="dft", max_iter=10000, tol=1e-12):
+    L = _setup()
+    g, h = _f64(g), _f64(hessian)
+    x = np.zeros_like(g) if x is None else _f64(x).copy()
+    N, _, X, Y, Z = g.shape
+    CC = 0 if h is None else h.shape[1]
+    it = L.or_cg_grid(_d(h), _d(g), _d(x), CC, N, X, Y, Z, _b(bound), _vs(voxel_size),
+                      *_pen(absolute, membrane, bending, lame), int(max_iter), float(tol))
+    return x, it

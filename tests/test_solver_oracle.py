@@ -1,0 +1,178 @@
+"""CPU tests of the solver oracle (FP64 restatement of regulariser_grid,
+relax_grid, FMG transfers and fmg_grid).
+
+Pinned by the reference's own vectors (tests/golden/solver_golden.npz, written
+by tests/golden/make_solver_golden.py from oracle/_ref) and by closed forms:
+plane-wave eigenvalues of the scalar penalties under circulant (dft) bounds,
+symmetry and positivity of (H + L), constant preservation of the transfer
+operators, and convergence of relax / FMG to the CG solution.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import solver as S
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden", "solver_golden.npz")
+BOUNDS = ["replicate", "dct1", "dct2", "dst1", "dst2", "dft", "zero"]
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+@pytest.fixture(scope="module")
+def golden():
+    g = np.load(GOLDEN)
+    keys = sorted({k.split("_")[0] for k in g.files})
+    return {k: {f.split("_", 1)[1]: g[f] for f in g.files if f.startswith(k + "_")} for k in keys}
+
+
+def test_regulariser_and_relax_golden(golden):
+    n = 0
+    for key, c in golden.items():
+        if not key.startswith("r"):
+            continue
+        b, CC = [int(v) for v in c["meta"]]
+        a, m, be, mu, la, *vs = [float(v) for v in c["pen"]]
+        kw = dict(absolute=a, membrane=m, bending=be, lame=(mu, la), voxel_size=vs, bound=BOUNDS[b])
+        h = c.get("h")
+        out = S.regulariser_grid(c["x"], h, **kw)
+        assert rel(out, c["reg"]) < 1e-12, key
+        if "relax" in c:
+            out = S.relax_grid(c["g"], h, c["x"], nb_iter=2, quirk_fold=True, **kw)
+            assert rel(out, c["relax"]) < 1e-12, key
+        n += 1
+    assert n >= 40
+
+
+def test_transfers_golden(golden):
+    for key, c in golden.items():
+        if not key.startswith("t"):
+            continue
+        b = BOUNDS[int(c["meta"][0])]
+        assert rel(S.restrict(c["fine"], c["restrict"].shape[2:], b), c["restrict"]) < 1e-13, key
+        assert rel(S.prolong(c["coarse"], c["prolong"].shape[2:], b), c["prolong"]) < 1e-13, key
+
+
+def test_fmg_golden(golden):
+    for key, c in golden.items():
+        if not key.startswith("f"):
+            continue
+        b = BOUNDS[int(c["meta"][0])]
+        a, m, be, mu, la = [float(v) for v in c["pen"]]
+        out = S.fmg_grid(c["h"], c["g"], absolute=a, membrane=m, bending=be, lame=(mu, la), bound=b,
+                         nb_cycles=2, nb_iter=2, quirk_fold=True, quirk_zscale=True)
+        assert rel(out, c["sol"]) < 1e-10, key
+
+
+@pytest.mark.parametrize("pen", [dict(absolute=0.3), dict(membrane=0.7, absolute=0.01),
+                                 dict(bending=0.2, membrane=0.01, absolute=1e-3)])
+def test_plane_wave_eigenvalues_dft(pen):
+    """With circulant bounds each component is a plane wave eigenvector of the
+    absolute/membrane/bending operator: L e_k = lam(k) e_k,
+    lam = v_c^2 (a + m S + b S^2) with S = sum_d (2 - 2 cos t_d) / v_d^2."""
+    shape = (8, 6, 10)
+    vs = np.array([1.0, 1.5, 0.7])
+    k = (3, 1, 2)
+    grids = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+    th = [2 * np.pi * k[d] / shape[d] for d in range(3)]
+    wave = np.cos(sum(th[d] * grids[d] for d in range(3)))
+    Ssym = sum((2 - 2 * np.cos(th[d])) / vs[d] ** 2 for d in range(3))
+    a, m, b = pen.get("absolute", 0), pen.get("membrane", 0), pen.get("bending", 0)
+    for c in range(3):
+        x = np.zeros((1, 3) + shape)
+        x[0, c] = wave
+        out = S.regulariser_grid(x, None, voxel_size=vs, bound="dft", **pen)
+        lam = vs[c] ** 2 * (a + m * Ssym + b * Ssym ** 2)
+        assert np.abs(out[0, c] - lam * wave).max() < 1e-10
+        others = [d for d in range(3) if d != c]
+        assert np.abs(out[0, others]).max() < 1e-12
+
+
+def test_lame_symbol_dft():
+    """Linear elasticity under circulant bounds: for a plane wave u = a cos(k.r),
+    L u = (mu |s|^2 a + (mu + lam) s (s.a)) cos(k.r) with the centred-difference
+    symbol of the discrete operator (unit voxels)."""
+    shape = (8, 8, 8)
+    k = (1, 2, 3)
+    grids = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+    th = np.array([2 * np.pi * k[d] / shape[d] for d in range(3)])
+    phase = sum(th[d] * grids[d] for d in range(3))
+    mu, lam = 0.3, 0.2
+    for amp in (np.array([1.0, 0, 0]), np.array([0, 1.0, 0.5])):
+        x = np.stack([a * np.cos(phase) for a in amp])[None]
+        out = S.regulariser_grid(x, None, lame=(mu, lam), bound="dft")
+        # second-difference symbols: D_dd = 2 - 2 cos t_d, mixed D_de = sin t_d sin t_e
+        Dd = 2 - 2 * np.cos(th)
+        M = np.zeros((3, 3))
+        for d in range(3):
+            M[d, d] = mu * (Dd.sum() + Dd[d]) + lam * Dd[d]
+            for e in range(3):
+                if e != d:
+                    M[d, e] = (mu + lam) * np.sin(th[d]) * np.sin(th[e])
+        ref = (M @ amp)[:, None, None, None] * np.cos(phase)[None]
+        assert np.abs(out[0] - ref).max() < 1e-10
+
+
+# With reflecting bounds the reference's Lame cross terms (mixed differences of
+# the other components) are not symmetric at the border, and replicate
+# (clamping) and dct1 (mirror about the edge voxel centre) are not symmetric
+# extensions under the plain inner product; the operator is symmetric for
+# circulant bounds, and for reflecting bounds without the Lame term.
+@pytest.mark.parametrize("bound,lame", [("dft", (0.05, 0.2)), ("dct2", 0), ("dst2", 0)])
+def test_operator_symmetric_positive(bound, lame):
+    rng = np.random.default_rng(2)
+    shape = (6, 7, 5)
+    h = rng.uniform(0.5, 2, (1, 6) + shape)
+    h[:, 3:] = rng.uniform(-0.2, 0.2, (1, 3) + shape)
+    kw = dict(absolute=1e-4, membrane=1e-3, bending=0.2, lame=lame, bound=bound)
+    u = rng.standard_normal((1, 3) + shape)
+    v = rng.standard_normal((1, 3) + shape)
+    Au = S.regulariser_grid(u, h, **kw)
+    Av = S.regulariser_grid(v, h, **kw)
+    assert abs(np.sum(Au * v) - np.sum(u * Av)) < 1e-10 * abs(np.sum(Au * v))
+    assert np.sum(Au * u) > 0
+
+
+def test_transfer_constants():
+    """Prolongation (quadratic, partition of unity) preserves constants for any
+    shape; restriction does for exact factors of 2 (its 4 linear weights
+    w(d/s)/s sum to 1 when s = 2)."""
+    for shape in [(8, 6, 10), (7, 5, 9)]:
+        c = np.full((1, 3) + shape, 2.5)
+        cs = tuple(1 + (s - 1) // 2 for s in shape)
+        if all(s % 2 == 0 for s in shape):
+            assert np.abs(S.restrict(c, cs, "dct2") - 2.5).max() < 1e-12
+        cc = np.full((1, 3) + cs, -1.5)
+        assert np.abs(S.prolong(cc, shape, "dct2") + 1.5).max() < 1e-12
+
+
+def test_relax_and_fmg_converge_to_cg():
+    rng = np.random.default_rng(3)
+    shape = (16, 12, 14)
+    g = rng.standard_normal((1, 3) + shape)
+    h = rng.uniform(0.5, 2, (1, 6) + shape)
+    h[:, 3:] = rng.uniform(-0.2, 0.2, (1, 3) + shape)
+    kw = dict(absolute=1e-4, membrane=1e-3, bending=0.2, lame=(0.05, 0.2), bound="dft")
+    xs, it = S.cg_grid(h, g, tol=1e-12, **kw)
+    r = g - S.regulariser_grid(xs, h, **kw)
+    assert np.linalg.norm(r) < 1e-10 * np.linalg.norm(g)
+    # Gauss-Seidel: residual decreases monotonically in energy and converges
+    x = np.zeros_like(g)
+    prev = np.inf
+    for _ in range(5):
+        x = S.relax_grid(g, h, x, nb_iter=4, **kw)
+        e = np.sum((x - xs) * S.regulariser_grid(x - xs, h, **kw))
+        assert e < prev
+        prev = e
+    # FMG with the intended per-channel scaling: more V-cycles, smaller
+    # residuals (the warm start only seeds the coarse levels: the finest
+    # solution is replaced by the prolonged coarse one, fmg_common.cpp:199)
+    res = []
+    for cycles in (1, 2, 4):
+        x = S.fmg_grid(h, g, nb_cycles=cycles, nb_iter=4, **kw)
+        res.append(np.linalg.norm(g - S.regulariser_grid(x, h, **kw)) / np.linalg.norm(g))
+    assert res[0] > res[1] > res[2] and res[2] < 1e-4
