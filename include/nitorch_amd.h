@@ -169,17 +169,20 @@ int ni_fmg_restriction(int32_t dtype, int64_t N, int64_t C, const int64_t *in_sh
 int ni_fmg_prolongation(int32_t dtype, int64_t N, int64_t C, const int64_t *in_shape, const int64_t *out_shape,
                         const void *input, void *output, const int32_t *bound, void *stream);
 
-/* Bytes of device workspace ni_fmg_grid needs for this problem. */
-int64_t ni_fmg_grid_workspace(int32_t dtype, int64_t N, const int64_t *shape, int32_t CC, int32_t max_levels);
+/* Bytes of device workspace ni_fmg_grid needs for this problem (use_cg:
+ * with the PCG smoother's scratch). */
+int64_t ni_fmg_grid_workspace(int32_t dtype, int64_t N, const int64_t *shape, int32_t CC, int32_t max_levels,
+                              int32_t use_cg);
 
-/* Replaces ni::fmg_grid (nitorch/_C/src/fmg.h:22-37, solver='relax'):
- * full multigrid with nb_cycles V-cycles per level and nb_iter relaxation
- * sweeps, then `extra_vcycles` further V-cycles on the finest level.
+/* Replaces ni::fmg_grid (nitorch/_C/src/fmg.h:22-37): full multigrid with
+ * nb_cycles V-cycles per level and nb_iter smoothing steps (use_cg = 0:
+ * Gauss-Seidel sweeps, solver='relax'; use_cg = 1: PCG iterations,
+ * solver='cg'), then `extra_vcycles` further V-cycles on the finest level.
  * `solution` is the initial guess (zeros for none) and receives the result. */
 int ni_fmg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
                 const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
                 const double *penalty, int32_t nb_cycles, int32_t nb_iter, int32_t max_levels,
-                int32_t extra_vcycles, void *workspace, int64_t workspace_bytes, void *stream);
+                int32_t use_cg, int32_t extra_vcycles, void *workspace, int64_t workspace_bytes, void *stream);
 
 /* Time-to-tolerance solve (no reference counterpart: the benchmark's "GN step
  * to 1e-4 relative residual"): ni_fmg_grid's full multigrid, then V-cycles on
@@ -190,8 +193,30 @@ int ni_fmg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hess
 int ni_fmg_solve(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
                  const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
                  const double *penalty, int32_t nb_cycles, int32_t nb_iter, int32_t max_levels,
-                 double tol, int32_t max_vcycles, double *norms, void *workspace, int64_t workspace_bytes,
-                 double *out_rel, int32_t *out_vcycles, void *stream);
+                 int32_t use_cg, double tol, int32_t max_vcycles, double *norms, void *workspace,
+                 int64_t workspace_bytes, double *out_rel, int32_t *out_vcycles, void *stream);
+
+/* Bytes of device workspace ni_pcg_grid needs (r, z, p and 4 N doubles). */
+int64_t ni_pcg_grid_workspace(int32_t dtype, int64_t N, const int64_t *shape);
+
+/* Replaces ni::pcg_grid (nitorch/_C/src/pcg.h:20-32, pcg_common.cpp:38-85,
+ * :125-163): nb_iter conjugate-gradient iterations on (H + L) x = g with the
+ * block-Jacobi preconditioner of ni_precond_grid; x updated in place.  tol = 0
+ * runs all iterations without host synchronisation; tol > 0 reads two
+ * scalars back per iteration and stops when
+ * sum_n alpha_n (alpha_n p'Ap + 2 r'z) < tol * N * prod(shape).
+ * out_iters (optional, host) receives the iterations done. */
+int ni_pcg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
+                const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
+                const double *penalty, int32_t nb_iter, double tol, void *workspace, int64_t workspace_bytes,
+                int32_t *out_iters, void *stream);
+
+/* Replaces ni::precond_grid (nitorch/_C/src/precond_grid.h:8-12,
+ * precond_grid_common.cpp:266-298, :620-660):
+ * output = (H (1 + 1e-6) + diag(L) (1 + 1e-6))^-1 gradient, voxel by voxel. */
+int ni_precond_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
+                    const void *gradient, void *output, const int32_t *bound, const double *voxel_size,
+                    const double *penalty, void *stream);
 
 #ifdef __cplusplus
 }

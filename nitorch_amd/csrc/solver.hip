@@ -24,9 +24,10 @@
 //    and the Lame cross terms; boundary folding is computed per axis (5
 //    offsets) and only branches at the border;
 //  * Gauss-Seidel uses 3x3x3 colour classes (bending, radius-2 stencil),
-//    2x2x2 (Lame only) or red/black (membrane only): each class is an
-//    independent set, so each launch is a race-free parallel update and the
-//    sweep is a true multi-colour Gauss-Seidel;
+//    2x2x2 (Lame only) or red/black (membrane only), with extra classes for
+//    the remainder slices of circulant axes: each class is an independent set,
+//    so each launch is a race-free parallel update and the sweep is a true
+//    multi-colour Gauss-Seidel;
 //  * levels with few voxels are relaxed by a single work-group that runs all
 //    sweeps and colours with barriers in between (one launch per relax);
 //  * restriction / prolongation gather (4^3 / 3^3 taps) with the per-channel
@@ -45,7 +46,7 @@
 namespace ni_amd {
 
 #define SV_BLOCK 256
-#define SV_SMALL 32768          // levels up to this many voxels: single-block relax
+#define SV_SMALL 16384          // levels up to this many voxels: single-block relax
 
 struct StencilW {
   double vx[3];        // 1 / voxel_size^2
@@ -260,25 +261,41 @@ __device__ __forceinline__ void relax_voxel(T *__restrict__ xn, const T *__restr
   xn[2 * V + i] = x2 + idt * (r0 * (p * s - q * b) + r1 * (p * q - a * s) + r2 * (a * b - p * p));
 }
 
-// One colour class of a multi-colour sweep: voxels with x % F == fx, etc.
-// (F = 3 for bending, 2 for Lame only).  bw == 0: red/black class `fx`.
+// Colour classes of a multi-colour sweep.  Along each axis a class is
+// {start + step * i, i < count}; a sweep visits every (x, y, z) class triple.
+// With period F (3 for bending's radius-2 stencil, 2 for Lame only) voxels of
+// a class are never coupled, so a class is updated in parallel without races.
+// Circulant axes whose length is not a multiple of F get one extra class per
+// remainder slice (otherwise the wrap-around couples two voxels of a class).
+struct AxisColours {
+  int n;
+  int start[8], step[8], count[8];
+};
+
+struct ColourPlan {
+  int redblack;            // 1: checkerboard (x + y + z) % 2 classes
+  AxisColours ax[3];
+};
+
+// One class triple (or one red/black class) of a sweep.
 template <typename T>
 __global__ __launch_bounds__(SV_BLOCK) void relax_colour(T *__restrict__ x, const T *__restrict__ g, const T *__restrict__ h,
-                                                         int CC, Geo G, StencilT<T> S, int N, int F,
-                                                         int fx, int fy, int fz, int nx, int ny, int nz) {
+                                                         int CC, Geo G, StencilT<T> S, int N, int redblack, int rb,
+                                                         int sx, int tx, int nx, int sy, int ty, int ny,
+                                                         int sz, int tz, int nz) {
   const int V = G.X * G.Y * G.Z;
   const long long gid = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
   int n, xx, y, z;
-  if (F > 0) {
+  if (!redblack) {
     const long long per = (long long)nx * ny * nz;
     if (gid >= per * N) return;
     n = (int)(gid / per);
     const int j = (int)(gid % per);
-    z = (j % nz) * F + fz;
-    y = ((j / nz) % ny) * F + fy;
-    xx = (j / (nz * ny)) * F + fx;
+    z = sz + tz * (j % nz);
+    y = sy + ty * ((j / nz) % ny);
+    xx = sx + tx * (j / (nz * ny));
   } else {
-    // red/black: class fx, half of each row along z
+    // red/black: class rb, half of each row along z
     const int nzh = (G.Z + 1) / 2;
     const long long per = (long long)G.X * G.Y * nzh;
     if (gid >= per * N) return;
@@ -287,38 +304,224 @@ __global__ __launch_bounds__(SV_BLOCK) void relax_colour(T *__restrict__ x, cons
     const int row = j / nzh;
     y = row % G.Y;
     xx = row / G.Y;
-    z = (j % nzh) * 2 + ((xx + y + fx) & 1);
+    z = (j % nzh) * 2 + ((xx + y + rb) & 1);
     if (z >= G.Z) return;
   }
   relax_voxel<T>(x + (size_t)n * 3 * V, g + (size_t)n * 3 * V, h ? h + (size_t)n * CC * V : nullptr,
                  CC, V, G, S, xx, y, z);
 }
 
-// All sweeps of a small level in one work-group (barriers between colours).
+// All sweeps of a small level in one work-group, with a barrier between
+// classes; threads enumerate the members of each class directly.
 template <typename T>
 __global__ __launch_bounds__(1024) void relax_small(T *__restrict__ x, const T *__restrict__ g, const T *__restrict__ h,
-                                                    int CC, Geo G, StencilT<T> S, int N, int F, int nb_iter) {
+                                                    int CC, Geo G, StencilT<T> S, int N, ColourPlan P, int nb_iter) {
   const int V = G.X * G.Y * G.Z;
-  const int nbr = F > 0 ? F * F * F : 2;
   for (int it = 0; it < nb_iter; ++it) {
-    for (int col = 0; col < nbr; ++col) {
-      for (long long k = threadIdx.x; k < (long long)N * V; k += blockDim.x) {
-        const int n = (int)(k / V), i = (int)(k % V);
-        const int z = i % G.Z, y = (i / G.Z) % G.Y, xx = i / (G.Y * G.Z);
-        bool mine;
-        if (F > 0) {
-          const int fx = col / (F * F), fy = (col / F) % F, fz = col % F;
-          mine = (xx % F == fx) && (y % F == fy) && (z % F == fz);
-        } else {
-          mine = ((xx + y + z) & 1) == col;
+    if (P.redblack) {
+      for (int rb = 0; rb < 2; ++rb) {
+        const int nzh = (G.Z + 1) / 2;
+        const long long per = (long long)G.X * G.Y * nzh;
+        for (long long k = threadIdx.x; k < per * N; k += blockDim.x) {
+          const int n = (int)(k / per), j = (int)(k % per);
+          const int row = j / nzh, y = row % G.Y, xx = row / G.Y;
+          const int z = (j % nzh) * 2 + ((xx + y + rb) & 1);
+          if (z < G.Z)
+            relax_voxel<T>(x + (size_t)n * 3 * V, g + (size_t)n * 3 * V, h ? h + (size_t)n * CC * V : nullptr,
+                           CC, V, G, S, xx, y, z);
         }
-        if (mine)
-          relax_voxel<T>(x + (size_t)n * 3 * V, g + (size_t)n * 3 * V, h ? h + (size_t)n * CC * V : nullptr,
-                         CC, V, G, S, xx, y, z);
+        __syncthreads();
       }
-      __syncthreads();
+      continue;
     }
+    for (int a = 0; a < P.ax[0].n; ++a)
+      for (int b = 0; b < P.ax[1].n; ++b)
+        for (int c = 0; c < P.ax[2].n; ++c) {
+          const int nx = P.ax[0].count[a], ny = P.ax[1].count[b], nz = P.ax[2].count[c];
+          const long long per = (long long)nx * ny * nz;
+          for (long long k = threadIdx.x; k < per * N; k += blockDim.x) {
+            const int n = (int)(k / per), j = (int)(k % per);
+            const int z = P.ax[2].start[c] + P.ax[2].step[c] * (j % nz);
+            const int y = P.ax[1].start[b] + P.ax[1].step[b] * ((j / nz) % ny);
+            const int xx = P.ax[0].start[a] + P.ax[0].step[a] * (j / (nz * ny));
+            relax_voxel<T>(x + (size_t)n * 3 * V, g + (size_t)n * 3 * V, h ? h + (size_t)n * CC * V : nullptr,
+                           CC, V, G, S, xx, y, z);
+          }
+          __syncthreads();
+        }
   }
+}
+
+// --------------------------------------------------------------------------
+// Preconditioned conjugate gradient (nitorch/_C/src/impl/pcg_common.cpp:38-85)
+// with the block-Jacobi preconditioner z = (H (1 + 1e-6) + diag(L)) \ r
+// (precond_grid_common.cpp:266-298, :620-660).  Scalars (r.z, p.Ap) stay on
+// the device, double precision, one slot per batch element and two
+// generations (ping-pong), so an iteration needs no host synchronisation.
+// --------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void block_jacobi(const T *__restrict__ hn, int CC, int V, int i, const StencilT<T> &S,
+                                             T v0, T v1, T v2, T &s0, T &s1, T &s2) {
+  T M[6];
+  load_H<T>(hn, hn ? CC : 0, V, i, M);
+  const T opt = T(1.000001);
+  const T a = M[0] * opt + S.diag[0], b = M[1] * opt + S.diag[1], c = M[2] * opt + S.diag[2];
+  const T p = M[3], q = M[4], s = M[5];
+  const T idt = T(1) / (a * b * c - a * s * s - b * q * q - c * p * p + T(2) * p * q * s);
+  s0 = idt * (v0 * (b * c - s * s) + v1 * (q * s - p * c) + v2 * (p * s - q * b));
+  s1 = idt * (v0 * (q * s - p * c) + v1 * (a * c - q * q) + v2 * (p * q - a * s));
+  s2 = idt * (v0 * (p * s - q * b) + v1 * (p * q - a * s) + v2 * (a * b - p * p));
+}
+
+// Per-batch double reduction of one value per thread into out[n].  A block
+// spans at most two batch elements when V >= SV_BLOCK (both are reduced in
+// the block, one atomic each); smaller levels fall back to per-thread atomics.
+// Summation order across blocks is that of the atomics (double precision).
+__device__ __forceinline__ void batch_reduce(double acc, int n, int V, double *__restrict__ out) {
+  if (V < SV_BLOCK) {
+    if (acc != 0) atomicAdd(out + n, acc);
+    return;
+  }
+  __shared__ double s_a[SV_BLOCK / 64], s_b[SV_BLOCK / 64];
+  const int n0 = (int)(((long long)blockIdx.x * SV_BLOCK) / V);
+  double a = n == n0 ? acc : 0.0, b = n == n0 ? 0.0 : acc;
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) { s_a[threadIdx.x >> 6] = a; s_b[threadIdx.x >> 6] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a = 0; b = 0;
+    for (int k = 0; k < SV_BLOCK / 64; ++k) { a += s_a[k]; b += s_b[k]; }
+    if (a != 0) atomicAdd(out + n0, a);
+    if (b != 0) atomicAdd(out + n0 + 1, b);
+  }
+}
+
+// r = g - (H + L) x ; z = M \ r ; p = z ; rz[n] += r.z
+template <typename T>
+__global__ __launch_bounds__(SV_BLOCK) void pcg_init(const T *__restrict__ x, const T *__restrict__ g,
+                                                     const T *__restrict__ h, int CC, T *__restrict__ r,
+                                                     T *__restrict__ z, T *__restrict__ p, Geo G, StencilT<T> S,
+                                                     int N, double *__restrict__ rz) {
+  const int V = G.X * G.Y * G.Z;
+  const long long gid = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
+  double acc = 0;
+  int n = 0;
+  if (gid < (long long)N * V) {
+    n = (int)(gid / V);
+    const int i = (int)(gid % V);
+    const int zz = i % G.Z, y = (i / G.Z) % G.Y, xx = i / (G.Y * G.Z);
+    const size_t b3 = (size_t)n * 3 * V;
+    T Lx[3];
+    apply_L<T>(x + b3, V, G, S, xx, y, zz, Lx);
+    const T *hn = h ? h + (size_t)n * CC * V : nullptr;
+    T M[6];
+    load_H<T>(hn, hn ? CC : 0, V, i, M);
+    const T x0 = x[b3 + i], x1 = x[b3 + V + i], x2 = x[b3 + 2 * V + i];
+    const T r0 = g[b3 + i] - (Lx[0] + M[0] * x0 + M[3] * x1 + M[4] * x2);
+    const T r1 = g[b3 + V + i] - (Lx[1] + M[3] * x0 + M[1] * x1 + M[5] * x2);
+    const T r2 = g[b3 + 2 * V + i] - (Lx[2] + M[4] * x0 + M[5] * x1 + M[2] * x2);
+    T z0, z1, z2;
+    block_jacobi<T>(hn, CC, V, i, S, r0, r1, r2, z0, z1, z2);
+    r[b3 + i] = r0; r[b3 + V + i] = r1; r[b3 + 2 * V + i] = r2;
+    z[b3 + i] = z0; z[b3 + V + i] = z1; z[b3 + 2 * V + i] = z2;
+    p[b3 + i] = z0; p[b3 + V + i] = z1; p[b3 + 2 * V + i] = z2;
+    acc = (double)r0 * z0 + (double)r1 * z1 + (double)r2 * z2;
+  }
+  batch_reduce(acc, n, V, rz);
+}
+
+// Ap = (H + L) p ; pap[n] += p.Ap
+template <typename T>
+__global__ __launch_bounds__(SV_BLOCK) void pcg_matvec(const T *__restrict__ p, const T *__restrict__ h, int CC,
+                                                       T *__restrict__ Ap, Geo G, StencilT<T> S, int N,
+                                                       double *__restrict__ pap) {
+  const int V = G.X * G.Y * G.Z;
+  const long long gid = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
+  double acc = 0;
+  int n = 0;
+  if (gid < (long long)N * V) {
+    n = (int)(gid / V);
+    const int i = (int)(gid % V);
+    const int zz = i % G.Z, y = (i / G.Z) % G.Y, xx = i / (G.Y * G.Z);
+    const size_t b3 = (size_t)n * 3 * V;
+    T Lp[3];
+    apply_L<T>(p + b3, V, G, S, xx, y, zz, Lp);
+    T M[6];
+    load_H<T>(h ? h + (size_t)n * CC * V : nullptr, h ? CC : 0, V, i, M);
+    const T p0 = p[b3 + i], p1 = p[b3 + V + i], p2 = p[b3 + 2 * V + i];
+    const T a0 = Lp[0] + M[0] * p0 + M[3] * p1 + M[4] * p2;
+    const T a1 = Lp[1] + M[3] * p0 + M[1] * p1 + M[5] * p2;
+    const T a2 = Lp[2] + M[4] * p0 + M[5] * p1 + M[2] * p2;
+    Ap[b3 + i] = a0; Ap[b3 + V + i] = a1; Ap[b3 + 2 * V + i] = a2;
+    acc = (double)p0 * a0 + (double)p1 * a1 + (double)p2 * a2;
+  }
+  batch_reduce(acc, n, V, pap);
+}
+
+// alpha = rz / max(pAp, 1e-12); x += alpha p; r -= alpha Ap; z = M \ r;
+// rz_new[n] += r.z      (Ap lives in z's storage on entry)
+template <typename T>
+__global__ __launch_bounds__(SV_BLOCK) void pcg_update(T *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
+                                                       const T *__restrict__ p, const T *__restrict__ h, int CC,
+                                                       Geo G, StencilT<T> S, int N, const double *__restrict__ rz,
+                                                       const double *__restrict__ pap, double *__restrict__ rz_new) {
+  const int V = G.X * G.Y * G.Z;
+  const long long gid = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
+  double acc = 0;
+  int n = 0;
+  if (gid < (long long)N * V) {
+    n = (int)(gid / V);
+    const int i = (int)(gid % V);
+    const size_t b3 = (size_t)n * 3 * V;
+    const double den = pap[n] > 1e-12 ? pap[n] : 1e-12;
+    const T alpha = T(rz[n] / den);
+    T rr[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const size_t k = b3 + d * V + i;
+      x[k] += alpha * p[k];
+      rr[d] = r[k] - alpha * z[k];
+      r[k] = rr[d];
+    }
+    T z0, z1, z2;
+    block_jacobi<T>(h ? h + (size_t)n * CC * V : nullptr, CC, V, i, S, rr[0], rr[1], rr[2], z0, z1, z2);
+    z[b3 + i] = z0; z[b3 + V + i] = z1; z[b3 + 2 * V + i] = z2;
+    acc = (double)rr[0] * z0 + (double)rr[1] * z1 + (double)rr[2] * z2;
+  }
+  batch_reduce(acc, n, V, rz_new);
+}
+
+// beta = rz_new / max(rz, 1e-12); p = z + beta p.  Also clears the two
+// accumulators the next iteration adds into (neither is read here).
+template <typename T>
+__global__ __launch_bounds__(SV_BLOCK) void pcg_direction(T *__restrict__ p, const T *__restrict__ z, int V, int N,
+                                                          const double *__restrict__ rz,
+                                                          const double *__restrict__ rz_new,
+                                                          double *__restrict__ clear_a, double *__restrict__ clear_b) {
+  const long long gid = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
+  if (gid < N) { clear_a[gid] = 0; clear_b[gid] = 0; }
+  if (gid >= (long long)N * 3 * V) return;
+  const int n = (int)(gid / (3LL * V));
+  const double den = rz[n] > 1e-12 ? rz[n] : 1e-12;
+  const T beta = T(rz_new[n] / den);
+  p[gid] = z[gid] + beta * p[gid];
+}
+
+// z = M \ g  (precond_grid)
+template <typename T>
+__global__ __launch_bounds__(SV_BLOCK) void precond_kernel(const T *__restrict__ g, const T *__restrict__ h, int CC,
+                                                           T *__restrict__ out, int V, int N, StencilT<T> S) {
+  const long long gid = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
+  if (gid >= (long long)N * V) return;
+  const int n = (int)(gid / V), i = (int)(gid % V);
+  const size_t b3 = (size_t)n * 3 * V;
+  T s0, s1, s2;
+  block_jacobi<T>(h ? h + (size_t)n * CC * V : nullptr, CC, V, i, S, g[b3 + i], g[b3 + V + i], g[b3 + 2 * V + i],
+                  s0, s1, s2);
+  out[b3 + i] = s0; out[b3 + V + i] = s1; out[b3 + 2 * V + i] = s2;
 }
 
 // --------------------------------------------------------------------------
@@ -430,34 +633,62 @@ static int relax_band(const double *pen) {
   return 1;
 }
 
+static void axis_colours(AxisColours &c, int n, int F, int bound) {
+  const int Fe = n < F ? n : F;
+  const int r = (bound == B_DFT && n > F) ? n % F : 0;
+  const int nF = n - r;
+  c.n = 0;
+  for (int k = 0; k < Fe; ++k) {
+    c.start[c.n] = k; c.step[c.n] = F; c.count[c.n] = (nF - k + F - 1) / F; c.n++;
+  }
+  for (int k = 0; k < r; ++k) {
+    c.start[c.n] = nF + k; c.step[c.n] = F; c.count[c.n] = 1; c.n++;
+  }
+}
+
+static ColourPlan make_plan(const Geo &G, int band) {
+  ColourPlan P;
+  std::memset(&P, 0, sizeof(P));
+  const int sz[3] = {G.X, G.Y, G.Z};
+  if (band == 0) {
+    // red/black unless a circulant axis has odd length (then 2x2x2 classes)
+    bool ok = true;
+    for (int d = 0; d < 3; ++d)
+      if (G.b[d] == B_DFT && (sz[d] & 1) && sz[d] > 1) ok = false;
+    if (ok) { P.redblack = 1; return P; }
+    band = 2;
+  }
+  for (int d = 0; d < 3; ++d) axis_colours(P.ax[d], sz[d], band, G.b[d]);
+  return P;
+}
+
 template <typename T>
 static void launch_relax(T *x, const T *g, const T *h, int CC, const Geo &G, const StencilT<T> &S,
                          int N, int band, int nb_iter, hipStream_t st) {
   const long long V = (long long)G.X * G.Y * G.Z;
+  const ColourPlan P = make_plan(G, band);
   if (V * N <= SV_SMALL) {
-    hipLaunchKernelGGL(relax_small<T>, dim3(1), dim3(1024), 0, st, x, g, h, CC, G, S, N, band, nb_iter);
+    hipLaunchKernelGGL(relax_small<T>, dim3(1), dim3(1024), 0, st, x, g, h, CC, G, S, N, P, nb_iter);
     return;
   }
   for (int it = 0; it < nb_iter; ++it) {
-    if (band == 0) {
+    if (P.redblack) {
       for (int rb = 0; rb < 2; ++rb) {
         const long long n = (long long)N * G.X * G.Y * ((G.Z + 1) / 2);
-        hipLaunchKernelGGL(relax_colour<T>, dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, x, g, h, CC, G, S, N, 0,
-                           rb, 0, 0, 0, 0, 0);
+        hipLaunchKernelGGL(relax_colour<T>, dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, x, g, h, CC, G, S, N, 1, rb,
+                           0, 0, 0, 0, 0, 0, 0, 0, 0);
       }
       continue;
     }
-    const int F = band;
-    for (int fx = 0; fx < F; ++fx)
-      for (int fy = 0; fy < F; ++fy)
-        for (int fz = 0; fz < F; ++fz) {
-          const int nx = fx < G.X ? (G.X - fx + F - 1) / F : 0;
-          const int ny = fy < G.Y ? (G.Y - fy + F - 1) / F : 0;
-          const int nz = fz < G.Z ? (G.Z - fz + F - 1) / F : 0;
-          const long long n = (long long)N * nx * ny * nz;
+    for (int a = 0; a < P.ax[0].n; ++a)
+      for (int b = 0; b < P.ax[1].n; ++b)
+        for (int c = 0; c < P.ax[2].n; ++c) {
+          const AxisColours &ax = P.ax[0], &ay = P.ax[1], &az = P.ax[2];
+          const long long n = (long long)N * ax.count[a] * ay.count[b] * az.count[c];
           if (n == 0) continue;
-          hipLaunchKernelGGL(relax_colour<T>, dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, x, g, h, CC, G, S, N, F,
-                             fx, fy, fz, nx, ny, nz);
+          hipLaunchKernelGGL(relax_colour<T>, dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, x, g, h, CC, G, S, N, 0, 0,
+                             ax.start[a], ax.step[a], ax.count[a], ay.start[b], ay.step[b], ay.count[b],
+                             az.start[c], az.step[c], az.count[c]);
         }
   }
 }
@@ -479,6 +710,46 @@ static void launch_transfer(const T *in, T *out, int N, int C, const int *si, co
     hipLaunchKernelGGL((transfer_kernel<T, 4>), dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, in, out, N * C, C, P, accumulate);
   else
     hipLaunchKernelGGL((transfer_kernel<T, 3>), dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, in, out, N * C, C, P, accumulate);
+}
+
+// PCG scratch: r, z, p (3 fields each) then 4 N doubles (rz ring of three, p.Ap)
+static size_t pcg_field_elems(int N, size_t V) { return (size_t)N * 3 * V; }
+static size_t align_up(size_t b, size_t a) { return (b + a - 1) / a * a; }
+
+// pcg_common.cpp:38-85.  tol == 0: nb_iter iterations, no host synchronisation;
+// tol > 0: each iteration reads p.Ap and r.z back and stops when
+// sum_n alpha (alpha pAp + 2 rz) < tol * N * V, before updating (as the reference).
+template <typename T>
+static int run_pcg(T *x, const T *g, const T *h, int CC, const Geo &G, const StencilT<T> &S, int N, int nb_iter,
+                   double tol, T *r, T *z, T *p, double *sc, hipStream_t st, int *iters) {
+  const long long V = (long long)G.X * G.Y * G.Z;
+  double *R[3] = {sc, sc + N, sc + 2 * N};
+  double *pap = sc + 3 * N;
+  launch_fill<double>(sc, 4LL * N, 0.0, st);
+  const unsigned bv = nblocks((long long)N * V), bf = nblocks((long long)N * 3 * V);
+  hipLaunchKernelGGL(pcg_init<T>, dim3(bv), dim3(SV_BLOCK), 0, st, x, g, h, CC, r, z, p, G, S, N, R[0]);
+  int it = 0;
+  std::vector<double> host(2 * (size_t)N);
+  for (; it < nb_iter; ++it) {
+    double *cur = R[it % 3], *nxt = R[(it + 1) % 3], *spare = R[(it + 2) % 3];
+    hipLaunchKernelGGL(pcg_matvec<T>, dim3(bv), dim3(SV_BLOCK), 0, st, p, h, CC, z, G, S, N, pap);
+    if (tol > 0) {
+      NI_HIP_CHECK(hipMemcpyAsync(host.data(), pap, N * sizeof(double), hipMemcpyDeviceToHost, st));
+      NI_HIP_CHECK(hipMemcpyAsync(host.data() + N, cur, N * sizeof(double), hipMemcpyDeviceToHost, st));
+      NI_HIP_CHECK(hipStreamSynchronize(st));
+      double obj = 0;
+      for (int n = 0; n < N; ++n) {
+        const double pa = host[n], rz = host[N + n];
+        const double alpha = rz / (pa > 1e-12 ? pa : 1e-12);
+        obj += alpha * (alpha * pa + 2 * rz);
+      }
+      if (obj < tol * (double)N * (double)V) break;
+    }
+    hipLaunchKernelGGL(pcg_update<T>, dim3(bv), dim3(SV_BLOCK), 0, st, x, r, z, p, h, CC, G, S, N, cur, pap, nxt);
+    hipLaunchKernelGGL(pcg_direction<T>, dim3(bf), dim3(SV_BLOCK), 0, st, p, z, (int)V, N, cur, nxt, spare, pap);
+  }
+  if (iters) *iters = it;
+  return 0;
 }
 
 // ------------------------------- FMG --------------------------------------
@@ -514,9 +785,16 @@ static size_t fmg_ws_elems(int N, int CC, const Level *L, int NL) {
   return e;
 }
 
+// bytes: pyramid, then (PCG smoother) z and p at level-0 size and 4 N doubles
+static size_t fmg_ws_bytes(int N, int CC, const Level *L, int NL, int use_cg, size_t esz) {
+  size_t b = fmg_ws_elems(N, CC, L, NL) * esz;
+  if (use_cg) b = align_up(b + 2 * pcg_field_elems(N, L[0].V) * esz, 256) + 4 * (size_t)N * sizeof(double);
+  return b;
+}
+
 template <typename T>
 struct FMG {
-  int N, CC, NL, band, nb_iter;
+  int N, CC, NL, band, nb_iter, use_cg;
   Level L[16];
   const T *h[16];
   const T *g0;
@@ -524,12 +802,21 @@ struct FMG {
   StencilT<T> S[16];
   Geo G[16];
   int bound[3];
+  T *cg_z, *cg_p;          // PCG smoother scratch (sized for level 0)
+  double *cg_sc;
   hipStream_t st;
 
   void factors(int l, float f[3]) const {  // fine / coarse size ratio between l and l+1
     for (int d = 0; d < 3; ++d) f[d] = (float)((double)L[l].sz[d] / (double)L[l + 1].sz[d]);
   }
-  void relax(int l) { launch_relax<T>(x[l], l == 0 ? g0 : g[l], h[l], CC, G[l], S[l], N, band, nb_iter, st); }
+  // fmg_common.cpp:375-387: Gauss-Seidel, or PCG when use_cg
+  void relax(int l) {
+    const T *gl = l == 0 ? g0 : g[l];
+    if (use_cg)
+      run_pcg<T>(x[l], gl, h[l], CC, G[l], S[l], N, nb_iter, 0.0, r[l], cg_z, cg_p, cg_sc, st, nullptr);
+    else
+      launch_relax<T>(x[l], gl, h[l], CC, G[l], S[l], N, band, nb_iter, st);
+  }
   void restrict_field(const T *fine, T *coarse, int l) {   // level l -> l+1, displacement scaling
     float f[3], cs[6] = {1, 1, 1, 1, 1, 1};
     factors(l, f);
@@ -574,14 +861,22 @@ struct FMG {
 template <typename T>
 static int fmg_setup(FMG<T> &F, int N, const int *sz, const T *hess, int CC, const T *grad, T *sol,
                      const int *bound, const double *vs, const double *pen, int nb_iter,
-                     int max_levels, void *ws, size_t ws_bytes, hipStream_t st) {
-  F.N = N; F.CC = hess ? CC : 0; F.nb_iter = nb_iter; F.st = st;
+                     int max_levels, int use_cg, void *ws, size_t ws_bytes, hipStream_t st) {
+  F.N = N; F.CC = hess ? CC : 0; F.nb_iter = nb_iter; F.use_cg = use_cg; F.st = st;
   for (int d = 0; d < 3; ++d) F.bound[d] = bound[d];
   F.NL = build_pyramid(sz, vs, max_levels, F.L);
   F.band = relax_band(pen);
-  const size_t need = fmg_ws_elems(N, F.CC, F.L, F.NL) * sizeof(T);
+  const size_t need = fmg_ws_bytes(N, F.CC, F.L, F.NL, use_cg, sizeof(T));
   NI_CHECK(ws_bytes >= need, "fmg_grid: workspace too small (%zu < %zu bytes)", ws_bytes, need);
   T *p = (T *)ws;
+  F.cg_z = F.cg_p = nullptr;
+  F.cg_sc = nullptr;
+  if (use_cg) {
+    const size_t pyr = fmg_ws_elems(N, F.CC, F.L, F.NL);
+    F.cg_z = p + pyr;
+    F.cg_p = F.cg_z + pcg_field_elems(N, F.L[0].V);
+    F.cg_sc = (double *)((char *)ws + align_up((pyr + 2 * pcg_field_elems(N, F.L[0].V)) * sizeof(T), 256));
+  }
   F.g0 = grad;
   F.h[0] = hess;
   F.x[0] = sol;
@@ -736,18 +1031,19 @@ int ni_fmg_prolongation(int32_t dtype, int64_t N, int64_t C, const int64_t *in_s
   return 0;
 }
 
-int64_t ni_fmg_grid_workspace(int32_t dtype, int64_t N, const int64_t *shape, int32_t CC, int32_t max_levels) {
+int64_t ni_fmg_grid_workspace(int32_t dtype, int64_t N, const int64_t *shape, int32_t CC, int32_t max_levels,
+                              int32_t use_cg) {
   Level L[16];
   const int sz[3] = {(int)shape[0], (int)shape[1], (int)shape[2]};
   const double vs[3] = {1, 1, 1};
   const int NL = build_pyramid(sz, vs, max_levels < 1 ? 1 : max_levels, L);
-  return (int64_t)(fmg_ws_elems((int)N, CC, L, NL) * (dtype == NI_FLOAT64 ? 8 : 4));
+  return (int64_t)fmg_ws_bytes((int)N, CC, L, NL, use_cg != 0, dtype == NI_FLOAT64 ? 8 : 4);
 }
 
 int ni_fmg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
                 const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
                 const double *penalty, int32_t nb_cycles, int32_t nb_iter, int32_t max_levels,
-                int32_t extra_vcycles, void *workspace, int64_t workspace_bytes, void *stream) {
+                int32_t use_cg, int32_t extra_vcycles, void *workspace, int64_t workspace_bytes, void *stream) {
   set_error("");
   NI_CHECK(gradient && solution && shape && bound && voxel_size && penalty && workspace, "fmg_grid: missing argument");
   const int sz[3] = {(int)shape[0], (int)shape[1], (int)shape[2]};
@@ -757,14 +1053,14 @@ int ni_fmg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hess
     FMG<float> F;
     if (int rc = fmg_setup<float>(F, (int)N, sz, (const float *)hessian, CC, (const float *)gradient,
                                   (float *)solution, bound, voxel_size, penalty, nb_iter, max_levels < 1 ? 1 : max_levels,
-                                  workspace, (size_t)workspace_bytes, st)) return rc;
+                                  use_cg != 0, workspace, (size_t)workspace_bytes, st)) return rc;
     fmg_run<float>(F, nb_cycles);
     for (int k = 0; k < extra_vcycles; ++k) F.vcycle(0);
   } else if (dtype == NI_FLOAT64) {
     FMG<double> F;
     if (int rc = fmg_setup<double>(F, (int)N, sz, (const double *)hessian, CC, (const double *)gradient,
                                    (double *)solution, bound, voxel_size, penalty, nb_iter, max_levels < 1 ? 1 : max_levels,
-                                   workspace, (size_t)workspace_bytes, st)) return rc;
+                                   use_cg != 0, workspace, (size_t)workspace_bytes, st)) return rc;
     fmg_run<double>(F, nb_cycles);
     for (int k = 0; k < extra_vcycles; ++k) F.vcycle(0);
   } else {
@@ -777,7 +1073,7 @@ int ni_fmg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hess
 int ni_fmg_solve(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
                  const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
                  const double *penalty, int32_t nb_cycles, int32_t nb_iter, int32_t max_levels,
-                 double tol, int32_t max_vcycles, double *norms, void *workspace, int64_t workspace_bytes,
+                 int32_t use_cg, double tol, int32_t max_vcycles, double *norms, void *workspace, int64_t workspace_bytes,
                  double *out_rel, int32_t *out_vcycles, void *stream) {
   set_error("");
   NI_CHECK(gradient && solution && shape && bound && voxel_size && penalty && workspace && norms,
@@ -807,19 +1103,83 @@ int ni_fmg_solve(int32_t dtype, int64_t N, const int64_t *shape, const void *hes
     FMG<float> F;
     if (int rc = fmg_setup<float>(F, (int)N, sz, (const float *)hessian, CC, (const float *)gradient,
                                   (float *)solution, bound, voxel_size, penalty, nb_iter, max_levels < 1 ? 1 : max_levels,
-                                  workspace, (size_t)workspace_bytes, st)) return rc;
+                                  use_cg != 0, workspace, (size_t)workspace_bytes, st)) return rc;
     if (int rc = loop(F)) return rc;
   } else if (dtype == NI_FLOAT64) {
     FMG<double> F;
     if (int rc = fmg_setup<double>(F, (int)N, sz, (const double *)hessian, CC, (const double *)gradient,
                                    (double *)solution, bound, voxel_size, penalty, nb_iter, max_levels < 1 ? 1 : max_levels,
-                                   workspace, (size_t)workspace_bytes, st)) return rc;
+                                   use_cg != 0, workspace, (size_t)workspace_bytes, st)) return rc;
     if (int rc = loop(F)) return rc;
   } else {
     NI_CHECK(false, "fmg_solve: unsupported dtype %d", dtype);
   }
   if (out_rel) *out_rel = rel;
   if (out_vcycles) *out_vcycles = cycles;
+  NI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int64_t ni_pcg_grid_workspace(int32_t dtype, int64_t N, const int64_t *shape) {
+  const size_t V = (size_t)shape[0] * shape[1] * shape[2];
+  return (int64_t)(align_up(3 * pcg_field_elems((int)N, V) * (dtype == NI_FLOAT64 ? 8 : 4), 256) +
+                   4 * (size_t)N * sizeof(double));
+}
+
+int ni_pcg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
+                const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
+                const double *penalty, int32_t nb_iter, double tol, void *workspace, int64_t workspace_bytes,
+                int32_t *out_iters, void *stream) {
+  set_error("");
+  NI_CHECK(gradient && solution && shape && bound && voxel_size && penalty && workspace, "pcg_grid: missing argument");
+  const int sz[3] = {(int)shape[0], (int)shape[1], (int)shape[2]};
+  if (int rc = fmg_check((int)N, sz, hessian ? CC : 0, bound, nb_iter)) return rc;
+  NI_CHECK(tol >= 0, "pcg_grid: tol must be >= 0");
+  const int64_t need = ni_pcg_grid_workspace(dtype, N, shape);
+  NI_CHECK(workspace_bytes >= need, "pcg_grid: workspace too small (%lld < %lld bytes)", (long long)workspace_bytes,
+           (long long)need);
+  Geo G;
+  G.X = sz[0]; G.Y = sz[1]; G.Z = sz[2];
+  for (int d = 0; d < 3; ++d) G.b[d] = bound[d];
+  const StencilW W = make_weights(voxel_size, penalty);
+  hipStream_t st = (hipStream_t)stream;
+  const size_t fe = pcg_field_elems((int)N, (size_t)sz[0] * sz[1] * sz[2]);
+  int iters = 0;
+  auto go = [&](auto *tag) -> int {
+    using T = typename std::remove_pointer<decltype(tag)>::type;
+    T *r = (T *)workspace, *z = r + fe, *p = z + fe;
+    double *sc = (double *)((char *)workspace + align_up(3 * fe * sizeof(T), 256));
+    return run_pcg<T>((T *)solution, (const T *)gradient, (const T *)hessian, hessian ? CC : 0, G,
+                      cast_weights<T>(W), (int)N, nb_iter, tol, r, z, p, sc, st, &iters);
+  };
+  int rc;
+  if (dtype == NI_FLOAT32) rc = go((float *)nullptr);
+  else if (dtype == NI_FLOAT64) rc = go((double *)nullptr);
+  else NI_CHECK(false, "pcg_grid: unsupported dtype %d", dtype);
+  if (rc) return rc;
+  if (out_iters) *out_iters = iters;
+  NI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int ni_precond_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
+                    const void *gradient, void *output, const int32_t *bound, const double *voxel_size,
+                    const double *penalty, void *stream) {
+  set_error("");
+  NI_CHECK(gradient && output && shape && bound && voxel_size && penalty, "precond_grid: missing argument");
+  const int sz[3] = {(int)shape[0], (int)shape[1], (int)shape[2]};
+  if (int rc = fmg_check((int)N, sz, hessian ? CC : 0, bound, 0)) return rc;
+  const StencilW W = make_weights(voxel_size, penalty);
+  const int V = sz[0] * sz[1] * sz[2];
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned nb = nblocks((long long)N * V);
+  if (dtype == NI_FLOAT32)
+    hipLaunchKernelGGL(precond_kernel<float>, dim3(nb), dim3(SV_BLOCK), 0, st, (const float *)gradient,
+                       (const float *)hessian, hessian ? CC : 0, (float *)output, V, (int)N, cast_weights<float>(W));
+  else if (dtype == NI_FLOAT64)
+    hipLaunchKernelGGL(precond_kernel<double>, dim3(nb), dim3(SV_BLOCK), 0, st, (const double *)gradient,
+                       (const double *)hessian, hessian ? CC : 0, (double *)output, V, (int)N, cast_weights<double>(W));
+  else NI_CHECK(false, "precond_grid: unsupported dtype %d", dtype);
   NI_HIP_CHECK(hipGetLastError());
   return 0;
 }

@@ -1,6 +1,7 @@
 """Displacement-field solver front-end, drop-in for the grid functions of
 `nitorch/_C/solve.py` (c_regulariser_grid :100-140, c_relax_grid :203-270,
-c_fmg_grid :574-641) and `nitorch.spatial.solve_grid_fmg`
+c_precond_grid :324-384, c_pcg_grid :445-505, c_fmg_grid :574-641) and
+`nitorch.spatial.solve_grid_fmg`
 (nitorch/spatial/_solvers.py:978-1039), running the HIP kernels of
 libnitorch_amd.so.
 
@@ -27,18 +28,30 @@ L.register_signatures({
     "ni_fmg_prolongation": [L._I32, L._I64, L._I64, L._PI64, L._PI64, L._P, L._P, L._PI32, L._P],
     "ni_fmg_grid": [L._I32, L._I64, L._PI64, L._P, L._I32, L._P, L._P, L._PI32,
                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
-                    L._I32, L._I32, L._I32, L._I32, L._P, L._I64, L._P],
+                    L._I32, L._I32, L._I32, L._I32, L._I32, L._P, L._I64, L._P],
     "ni_fmg_solve": [L._I32, L._I64, L._PI64, L._P, L._I32, L._P, L._P, L._PI32,
                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
-                     L._I32, L._I32, L._I32, ctypes.c_double, L._I32, L._P, L._P, L._I64,
+                     L._I32, L._I32, L._I32, L._I32, ctypes.c_double, L._I32, L._P, L._P, L._I64,
                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32), L._P],
+    "ni_pcg_grid": [L._I32, L._I64, L._PI64, L._P, L._I32, L._P, L._P, L._PI32,
+                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                    L._I32, ctypes.c_double, L._P, L._I64, ctypes.POINTER(ctypes.c_int32), L._P],
+    "ni_precond_grid": [L._I32, L._I64, L._PI64, L._P, L._I32, L._P, L._P, L._PI32,
+                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), L._P],
 })
 
 
 def _ws_fn():
     fn = L.lib().ni_fmg_grid_workspace
     fn.restype = ctypes.c_int64
-    fn.argtypes = [L._I32, L._I64, L._PI64, L._I32, L._I32]
+    fn.argtypes = [L._I32, L._I64, L._PI64, L._I32, L._I32, L._I32]
+    return fn
+
+
+def _pcg_ws_fn():
+    fn = L.lib().ni_pcg_grid_workspace
+    fn.restype = ctypes.c_int64
+    fn.argtypes = [L._I32, L._I64, L._PI64]
     return fn
 
 
@@ -50,8 +63,8 @@ def _dvec(vals):
 def _penalty(absolute, membrane, bending, lame, factor):
     lame = make_list(lame) if lame is not None else [0.0]
     lame = (lame + [lame[-1]])[:2]
-    return _dvec([absolute * factor, membrane * factor, bending * factor,
-                  lame[0] * factor, lame[1] * factor])
+    return [absolute * factor, membrane * factor, bending * factor,
+            lame[0] * factor, lame[1] * factor]
 
 
 def _voxel(voxel_size):
@@ -157,20 +170,51 @@ def fmg_prolongation(input, output=None, bound="dct2"):
 
 
 def fmg_grid_cf(hessian, g, x=None, penalty=(0, 0, 0, 0, 0), voxel_size=(1, 1, 1), bound="dft",
-                nb_cycles=2, nb_iter=2, max_levels=16, extra_vcycles=0, workspace=None):
-    """Full multigrid on contiguous channel-first tensors; x updated in place."""
+                nb_cycles=2, nb_iter=2, max_levels=16, extra_vcycles=0, workspace=None,
+                use_cg=False):
+    """Full multigrid on contiguous channel-first tensors; x updated in place.
+    use_cg: PCG smoothing (the reference's solver='cg'), else Gauss-Seidel."""
     x = torch.zeros_like(g) if x is None else x
     CC = 0 if hessian is None else hessian.shape[1]
     dt = L.dtype_code(g.dtype)
-    nbytes = _ws_fn()(dt, g.shape[0], L.i64arr(g.shape[2:]), CC, int(max_levels))
+    nbytes = _ws_fn()(dt, g.shape[0], L.i64arr(g.shape[2:]), CC, int(max_levels), int(bool(use_cg)))
     if workspace is None or workspace.numel() < nbytes:
         workspace = torch.empty(int(nbytes), dtype=torch.uint8, device=g.device)
     L.check(L.lib().ni_fmg_grid(
         dt, g.shape[0], L.i64arr(g.shape[2:]), L.ptr(hessian), CC, L.ptr(g), L.ptr(x),
         _bounds(bound), _voxel(voxel_size), _dvec(penalty), int(nb_cycles), int(nb_iter),
-        int(max_levels), int(extra_vcycles), L.ptr(workspace), int(nbytes), L.stream_of(g)),
-        "fmg_grid")
+        int(max_levels), int(bool(use_cg)), int(extra_vcycles), L.ptr(workspace), int(nbytes),
+        L.stream_of(g)), "fmg_grid")
     return x
+
+
+def pcg_grid_cf(hessian, g, x=None, penalty=(0, 0, 0, 0, 0), voxel_size=(1, 1, 1), bound="dft",
+                nb_iter=16, tol=0.0, workspace=None):
+    """Block-Jacobi preconditioned CG (nitorch/_C/src/impl/pcg_common.cpp:38-85) on
+    contiguous channel-first tensors; x updated in place.  Returns (x, iterations)."""
+    x = torch.zeros_like(g) if x is None else x
+    CC = 0 if hessian is None else hessian.shape[1]
+    dt = L.dtype_code(g.dtype)
+    nbytes = _pcg_ws_fn()(dt, g.shape[0], L.i64arr(g.shape[2:]))
+    if workspace is None or workspace.numel() < nbytes:
+        workspace = torch.empty(int(nbytes), dtype=torch.uint8, device=g.device)
+    iters = ctypes.c_int32(0)
+    L.check(L.lib().ni_pcg_grid(
+        dt, g.shape[0], L.i64arr(g.shape[2:]), L.ptr(hessian), CC, L.ptr(g), L.ptr(x),
+        _bounds(bound), _voxel(voxel_size), _dvec(penalty), int(nb_iter), float(tol),
+        L.ptr(workspace), int(nbytes), ctypes.byref(iters), L.stream_of(g)), "pcg_grid")
+    return x, iters.value
+
+
+def precond_grid_cf(hessian, g, penalty=(0, 0, 0, 0, 0), voxel_size=(1, 1, 1), bound="dft", out=None):
+    """(H + diag L)^-1 g voxel by voxel (precond_grid_common.cpp:620-660)."""
+    out = torch.empty_like(g) if out is None else out
+    CC = 0 if hessian is None else hessian.shape[1]
+    L.check(L.lib().ni_precond_grid(
+        L.dtype_code(g.dtype), g.shape[0], L.i64arr(g.shape[2:]), L.ptr(hessian), CC, L.ptr(g),
+        L.ptr(out), _bounds(bound), _voxel(voxel_size), _dvec(penalty), L.stream_of(g)),
+        "precond_grid")
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -186,7 +230,7 @@ def regulariser_grid(input, weight=None, hessian=None, absolute=0, membrane=0, b
     x = _to_cf(input, "input")
     h = _to_cf(hessian, "hessian")
     out = regulariser_grid_cf(x, h, _penalty(absolute, membrane, bending, lame, factor),
-                              _voxel(voxel_size), bound)
+                              voxel_size, bound)
     out = out.movedim(1, -1)
     if output is not None:
         output.copy_(out.reshape(output.shape))
@@ -206,7 +250,7 @@ def relax_grid(gradient, weight=None, hessian=None, absolute=0, membrane=0, bend
     g = _to_cf(gradient, "gradient")
     h = _to_cf(hessian, "hessian")
     x = torch.zeros_like(g) if output is None else _to_cf(output, "output")
-    relax_grid_cf(g, x, h, _penalty(absolute, membrane, bending, lame, factor), _voxel(voxel_size),
+    relax_grid_cf(g, x, h, _penalty(absolute, membrane, bending, lame, factor), voxel_size,
                   bound, nb_iter)
     res = x.movedim(1, -1)
     if output is not None:
@@ -217,24 +261,7 @@ def relax_grid(gradient, weight=None, hessian=None, absolute=0, membrane=0, bend
     return res
 
 
-def fmg_grid(hessian, gradient, weight=None, absolute=0, membrane=0, bending=0, lame=0, factor=1,
-             voxel_size=1, bound="dft", nb_cycles=2, nb_iter=2, max_levels=16, solver="relax",
-             output=None):
-    """Full multigrid solve of (H + L) x = g (nitorch/_C/solve.py c_fmg_grid).
-
-    solver: 'relax' (Gauss-Seidel smoothing, the GPU path).  The reference's
-    'cg' smoother is not provided; asking for it raises.
-    """
-    _no_weight(weight)
-    if solver not in ("relax", "relaks"):
-        raise NotImplementedError("fmg_grid: only solver='relax' (Gauss-Seidel) is provided")
-    _check(gradient, hessian)
-    pad = 5 - gradient.dim()
-    g = _to_cf(gradient, "gradient")
-    h = _to_cf(hessian, "hessian")
-    x = torch.zeros_like(g) if output is None else _to_cf(output, "output")
-    fmg_grid_cf(h, g, x, _penalty(absolute, membrane, bending, lame, factor), _voxel(voxel_size),
-                bound, nb_cycles, nb_iter, max_levels)
+def _finish(x, output, pad):
     res = x.movedim(1, -1)
     if output is not None:
         output.copy_(res.reshape(output.shape))
@@ -244,8 +271,59 @@ def fmg_grid(hessian, gradient, weight=None, absolute=0, membrane=0, bending=0, 
     return res
 
 
+def fmg_grid(hessian, gradient, weight=None, absolute=0, membrane=0, bending=0, lame=0, factor=1,
+             voxel_size=1, bound="dft", nb_cycles=2, nb_iter=2, max_levels=16, solver="cg",
+             output=None):
+    """Full multigrid solve of (H + L) x = g (nitorch/_C/solve.py c_fmg_grid).
+
+    solver: 'cg' (block-Jacobi PCG smoothing, the reference's default) or
+    'relax' (multi-colour Gauss-Seidel).
+    """
+    _no_weight(weight)
+    if solver == "relaks":
+        solver = "relax"
+    if solver not in ("relax", "cg"):
+        raise ValueError(f"fmg_grid: solver must be 'cg' or 'relax', got {solver!r}")
+    _check(gradient, hessian)
+    pad = 5 - gradient.dim()
+    g = _to_cf(gradient, "gradient")
+    h = _to_cf(hessian, "hessian")
+    x = torch.zeros_like(g) if output is None else _to_cf(output, "output")
+    fmg_grid_cf(h, g, x, _penalty(absolute, membrane, bending, lame, factor), voxel_size,
+                bound, nb_cycles, nb_iter, max_levels, use_cg=(solver == "cg"))
+    return _finish(x, output, pad)
+
+
+def pcg_grid(gradient, weight=None, hessian=None, absolute=0, membrane=0, bending=0, lame=0,
+             factor=1, voxel_size=1, bound="dft", nb_iter=16, tol=1e-4, output=None):
+    """Preconditioned conjugate gradient on (H + L) x = g
+    (nitorch/_C/solve.py c_pcg_grid); `output` is the initial guess."""
+    _no_weight(weight)
+    _check(gradient, hessian)
+    pad = 5 - gradient.dim()
+    g = _to_cf(gradient, "gradient")
+    h = _to_cf(hessian, "hessian")
+    x = torch.zeros_like(g) if output is None else _to_cf(output, "output")
+    pcg_grid_cf(h, g, x, _penalty(absolute, membrane, bending, lame, factor), voxel_size, bound,
+                nb_iter, tol)
+    return _finish(x, output, pad)
+
+
+def precond_grid(gradient, weight=None, hessian=None, absolute=0, membrane=0, bending=0, lame=0,
+                 factor=1, voxel_size=1, bound="dft", output=None):
+    """Block-Jacobi preconditioner (H + diag L)^-1 g
+    (nitorch/_C/solve.py c_precond_grid)."""
+    _no_weight(weight)
+    _check(gradient, hessian)
+    pad = 5 - gradient.dim()
+    g = _to_cf(gradient, "gradient")
+    h = _to_cf(hessian, "hessian")
+    out = precond_grid_cf(h, g, _penalty(absolute, membrane, bending, lame, factor), voxel_size, bound)
+    return _finish(out, output, pad)
+
+
 def solve_grid_fmg(hessian, gradient, absolute=0, membrane=0, bending=0, lame=0, factor=1,
-                   voxel_size=1, bound="dft", weights=None, optim="relax", nb_cycles=2, nb_iter=2):
+                   voxel_size=1, bound="dft", weights=None, optim="cg", nb_cycles=2, nb_iter=2):
     """nitorch.spatial.solve_grid_fmg (nitorch/spatial/_solvers.py:978-1039) on the GPU.
     Without spatial penalties the system is voxel-wise and solved in closed form."""
     _no_weight(weights)
@@ -281,7 +359,7 @@ def solve_grid_closedform(hessian, gradient, absolute=0, factor=1, voxel_size=1)
 
 def solve_to_tolerance(hessian, gradient, penalty, voxel_size=(1, 1, 1), bound="dft",
                        tol=1e-4, nb_cycles=2, nb_iter=4, max_vcycles=20, max_levels=16, x=None,
-                       workspace=None, norms=None):
+                       workspace=None, norms=None, use_cg=False):
     """Plain GPU FMG followed by V-cycles on the finest level until
     ||g - (H + L) x|| / ||g|| <= tol.  Channel-first contiguous tensors.
     Returns (x, relative residual, number of extra V-cycles)."""
@@ -289,7 +367,7 @@ def solve_to_tolerance(hessian, gradient, penalty, voxel_size=(1, 1, 1), bound="
     x = torch.zeros_like(g) if x is None else x
     CC = 0 if h is None else h.shape[1]
     dt = L.dtype_code(g.dtype)
-    nbytes = _ws_fn()(dt, g.shape[0], L.i64arr(g.shape[2:]), CC, int(max_levels))
+    nbytes = _ws_fn()(dt, g.shape[0], L.i64arr(g.shape[2:]), CC, int(max_levels), int(bool(use_cg)))
     if workspace is None or workspace.numel() < nbytes:
         workspace = torch.empty(int(nbytes), dtype=torch.uint8, device=g.device)
     if norms is None:
@@ -299,6 +377,6 @@ def solve_to_tolerance(hessian, gradient, penalty, voxel_size=(1, 1, 1), bound="
     L.check(L.lib().ni_fmg_solve(
         dt, g.shape[0], L.i64arr(g.shape[2:]), L.ptr(h), CC, L.ptr(g), L.ptr(x), _bounds(bound),
         _voxel(voxel_size), _dvec(penalty), int(nb_cycles), int(nb_iter), int(max_levels),
-        float(tol), int(max_vcycles), L.ptr(norms), L.ptr(workspace), int(nbytes),
+        int(bool(use_cg)), float(tol), int(max_vcycles), L.ptr(norms), L.ptr(workspace), int(nbytes),
         ctypes.byref(rel), ctypes.byref(ncyc), L.stream_of(g)), "fmg_solve")
     return x, rel.value, ncyc.value

@@ -27,10 +27,14 @@ def _setup():
     L.or_prolong.restype = None
     L.or_restrict.argtypes = [D, D, I64, I64, PI64, PI64, PI]
     L.or_restrict.restype = None
-    L.or_fmg_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5 + [I, I, I, I, I]
+    L.or_fmg_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5 + [I, I, I, I, I, I]
     L.or_fmg_grid.restype = None
     L.or_cg_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5 + [I, _c.c_double]
     L.or_cg_grid.restype = I
+    L.or_pcg_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5 + [I, _c.c_double]
+    L.or_pcg_grid.restype = I
+    L.or_precond_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, D] + [_c.c_double] * 5
+    L.or_precond_grid.restype = None
     L._solver_ready = True
     return L
 
@@ -73,14 +77,17 @@ def regulariser_grid(x, hessian=None, absolute=0, membrane=0, bending=0, lame=0,
 
 
 def relax_grid(g, hessian=None, x=None, absolute=0, membrane=0, bending=0, lame=0,
-               voxel_size=1, bound="dft", nb_iter=1, quirk_fold=False):
+               voxel_size=1, bound="dft", nb_iter=1, quirk_fold=False, product_order=False):
+    """Gauss-Seidel sweeps.  quirk_fold: the reference's fold order and sign
+    typos; product_order: the wrap-safe colour classes the HIP kernels use."""
     L = _setup()
     g, h = _f64(g), _f64(hessian)
     x = np.zeros_like(g) if x is None else _f64(x).copy()
     N, _, X, Y, Z = g.shape
     CC = 0 if h is None else h.shape[1]
+    mode = 2 if product_order else int(bool(quirk_fold))
     L.or_relax_grid(_d(g), _d(h), CC, _d(x), N, X, Y, Z, _b(bound), _vs(voxel_size),
-                    *_pen(absolute, membrane, bending, lame), int(nb_iter), int(quirk_fold))
+                    *_pen(absolute, membrane, bending, lame), int(nb_iter), mode)
     return x
 
 
@@ -105,7 +112,8 @@ def restrict(x, out_shape=None, bound="dct2"):
 
 
 def fmg_grid(hessian, g, x=None, absolute=0, membrane=0, bending=0, lame=0, voxel_size=1,
-             bound="dft", nb_cycles=2, nb_iter=2, max_levels=16, quirk_fold=False, quirk_zscale=False):
+             bound="dft", nb_cycles=2, nb_iter=2, max_levels=16, quirk_fold=False, quirk_zscale=False,
+             product_order=False, use_cg=False):
     L = _setup()
     g, h = _f64(g), _f64(hessian)
     x = np.zeros_like(g) if x is None else _f64(x).copy()
@@ -113,8 +121,34 @@ def fmg_grid(hessian, g, x=None, absolute=0, membrane=0, bending=0, lame=0, voxe
     CC = 0 if h is None else h.shape[1]
     L.or_fmg_grid(_d(h), _d(g), _d(x), CC, N, X, Y, Z, _b(bound), _vs(voxel_size),
                   *_pen(absolute, membrane, bending, lame), int(nb_cycles), int(nb_iter),
-                  int(max_levels), int(quirk_fold), int(quirk_zscale))
+                  int(max_levels), 2 if product_order else int(bool(quirk_fold)), int(quirk_zscale),
+                  int(bool(use_cg)))
     return x
+
+
+def precond_grid(g, hessian=None, absolute=0, membrane=0, bending=0, lame=0, voxel_size=1):
+    """(H (1+1e-6) + diag(L) (1+1e-6))^-1 g, voxel by voxel."""
+    L = _setup()
+    g, h = _f64(g), _f64(hessian)
+    out = np.empty_like(g)
+    N, _, X, Y, Z = g.shape
+    CC = 0 if h is None else h.shape[1]
+    L.or_precond_grid(_d(h), _d(g), _d(out), CC, N, X, Y, Z, _vs(voxel_size),
+                      *_pen(absolute, membrane, bending, lame))
+    return out
+
+
+def pcg_grid(hessian, g, x=None, absolute=0, membrane=0, bending=0, lame=0, voxel_size=1,
+             bound="dft", nb_iter=16, tol=0.0):
+    """Preconditioned CG as the reference's do_pcg; returns (x, iterations)."""
+    L = _setup()
+    g, h = _f64(g), _f64(hessian)
+    x = np.zeros_like(g) if x is None else _f64(x).copy()
+    N, _, X, Y, Z = g.shape
+    CC = 0 if h is None else h.shape[1]
+    it = L.or_pcg_grid(_d(h), _d(g), _d(x), CC, N, X, Y, Z, _b(bound), _vs(voxel_size),
+                       *_pen(absolute, membrane, bending, lame), int(nb_iter), float(tol))
+    return x, it
 
 
 def cg_grid(hessian, g, x=None, absolute=0, membrane=0, bending=0, lame=0, voxel_size=1,

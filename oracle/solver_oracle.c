@@ -12,15 +12,21 @@
  *   or_prolong / or_restrict : FMG transfer operators
  *       (multires_common.cpp: resize3d :430-499, restrict3d :586-669,
  *        shapes :800-839)
- *   or_fmg_grid         : full multigrid (fmg_common.cpp:45-222, :356-533)
+ *   or_precond_grid     : block-Jacobi solve (H + diag L)^-1 g
+ *       (precond_grid_common.cpp, set_kernel :266-298, invert3d_* :620-700)
+ *   or_pcg_grid         : preconditioned CG (pcg_common.cpp:38-85, :125-163)
+ *   or_fmg_grid         : full multigrid (fmg_common.cpp:45-222, :356-533),
+ *                         Gauss-Seidel or PCG smoother (:375-387)
  *   or_cg_grid          : plain CG on (H + L) x = g (exact solutions for tests)
  *
  * Quirk switches reproduce the reference's own behaviour where it departs
  * from the intended algorithm, so that the restatement can be compared
  * exactly with the compiled reference:
- *   quirk_fold  : fold start indices of banded relax use `% Y`, `% Z`
+ *   quirk_fold  : 1 = fold start indices of banded relax use `% Y`, `% Z`
  *                 (relax_grid_common.cpp:469-470), and the sign typos of
  *                 relax3d_all's third component (:1154-1158) are kept;
+ *                 0 = the reference's colour order without those quirks;
+ *                 2 = the product's wrap-safe colour classes (axis_colours);
  *   quirk_zscale: FMG rescales Z-slices instead of displacement channels
  *                 (fmg_common.cpp:392-489).
  * With both off, the restatement follows the intended semantics (per-channel
@@ -241,6 +247,28 @@ static void relax_voxel(double *x3, const double *g3, const double *h, int CC, c
   x3[i] += d0; x3[V + i] += d1; x3[2 * V + i] += d2;
 }
 
+/* Colour classes along one axis of length n for period F: class c covers
+ * start + F*i.  With circulant bounds and n % F != 0 the wrap-around would put
+ * two coupled voxels in one class, so the last n % F slices get classes of
+ * their own (the product's ordering; the reference uses plain x % F). */
+typedef struct { int64_t start[8], step[8], count[8]; int n; } colours_t;
+
+static void axis_colours(colours_t *c, int64_t n, int64_t F, int bound, int safe)
+{
+  int64_t Fe = n < F ? n : F;
+  int64_t r = (safe && bound == B_DFT && n > F) ? n % F : 0;
+  int64_t nF = n - r;
+  c->n = 0;
+  for (int64_t k = 0; k < Fe; ++k) {
+    c->start[c->n] = k; c->step[c->n] = F; c->count[c->n] = (nF - k + F - 1) / F; c->n++;
+  }
+  for (int64_t k = 0; k < r; ++k) {
+    c->start[c->n] = nF + k; c->step[c->n] = F; c->count[c->n] = 1; c->n++;
+  }
+}
+
+/* order: 0 = the reference's folds (quirk_fold reproduces its start indices),
+ *        1 = the product's wrap-safe colour classes (see axis_colours). */
 void or_relax_grid(const double *g, const double *h, int CC, double *x,
                    int64_t N, int64_t X, int64_t Y, int64_t Z, const int *bound,
                    const double *voxel_size, double absolute, double membrane,
@@ -250,8 +278,17 @@ void or_relax_grid(const double *g, const double *h, int CC, double *x,
   weights_t W; set_weights(&W, voxel_size, absolute, membrane, bending, lame_shear, lame_div);
   geom_t G = {X, Y, Z, {bound[0], bound[1], bound[2]}};
   int64_t V = X * Y * Z;
+  int safe = quirk_fold == 2;           /* 2: product ordering */
+  int quirk = quirk_fold == 1;
   /* bandwidth: bending 3, Lame 2, membrane 0 (red/black), else 1 */
   int bw = bending ? 3 : ((lame_shear || lame_div) ? 2 : (membrane ? 0 : 1));
+  if (safe && bw == 0) {
+    /* red/black is not an independent set when a circulant axis is odd */
+    for (int d = 0; d < 3; ++d) {
+      int64_t n = d == 0 ? X : (d == 1 ? Y : Z);
+      if (bound[d] == B_DFT && (n & 1) && n > 1) bw = 2;
+    }
+  }
   for (int64_t n = 0; n < N; ++n) {
     double *xn = x + n * 3 * V;
     const double *gn = g + n * 3 * V;
@@ -262,18 +299,33 @@ void or_relax_grid(const double *g, const double *h, int CC, double *x,
           for (int64_t xx = 0; xx < X; ++xx)
             for (int64_t yy = 0; yy < Y; ++yy)
               for (int64_t zz = 0; zz < Z; ++zz)
-                if ((xx + yy + zz) % 2 == rb) relax_voxel(xn, gn, hn, CC, &G, &W, V, xx, yy, zz, quirk_fold);
+                if ((xx + yy + zz) % 2 == rb) relax_voxel(xn, gn, hn, CC, &G, &W, V, xx, yy, zz, quirk);
+        continue;
+      }
+      if (safe) {
+        colours_t cx, cy, cz;
+        axis_colours(&cx, X, bw, bound[0], 1);
+        axis_colours(&cy, Y, bw, bound[1], 1);
+        axis_colours(&cz, Z, bw, bound[2], 1);
+        for (int a = 0; a < cx.n; ++a)
+          for (int b = 0; b < cy.n; ++b)
+            for (int c = 0; c < cz.n; ++c)
+              for (int64_t i = 0; i < cx.count[a]; ++i)
+                for (int64_t j = 0; j < cy.count[b]; ++j)
+                  for (int64_t k = 0; k < cz.count[c]; ++k)
+                    relax_voxel(xn, gn, hn, CC, &G, &W, V, cx.start[a] + cx.step[a] * i,
+                                cy.start[b] + cy.step[b] * j, cz.start[c] + cz.step[c] * k, 0);
         continue;
       }
       int64_t Fx = X < bw ? X : bw, Fy = Y < bw ? Y : bw, Fz = Z < bw ? Z : bw;
       for (int64_t f = 0; f < Fx * Fy * Fz; ++f) {
         int64_t fx = f / (Fy * Fz);
-        int64_t fy = quirk_fold ? (f / Fz) % Y : (f / Fz) % Fy;
-        int64_t fz = quirk_fold ? f % Z : f % Fz;
+        int64_t fy = quirk ? (f / Fz) % Y : (f / Fz) % Fy;
+        int64_t fz = quirk ? f % Z : f % Fz;
         for (int64_t xx = fx; xx < X; xx += Fx)
           for (int64_t yy = fy; yy < Y; yy += Fy)
             for (int64_t zz = fz; zz < Z; zz += Fz)
-              relax_voxel(xn, gn, hn, CC, &G, &W, V, xx, yy, zz, quirk_fold);
+              relax_voxel(xn, gn, hn, CC, &G, &W, V, xx, yy, zz, quirk);
       }
     }
   }
@@ -381,11 +433,110 @@ static void scale_fields(double *t, int64_t N, int64_t C, const int64_t *sz, con
   }
 }
 
+/* (H (1+tiny) + diag(L) (1+tiny))^-1 g per voxel: the symmetric 3x3 solve by
+ * the adjugate for CC = 6, divisions for CC in {0, 1, 3}
+ * (precond_grid_common.cpp:620-700). */
+void or_precond_grid(const double *h, const double *g, double *out, int CC,
+                     int64_t N, int64_t X, int64_t Y, int64_t Z, const double *voxel_size,
+                     double absolute, double membrane, double bending, double lame_shear,
+                     double lame_div)
+{
+  weights_t W; set_weights(&W, voxel_size, absolute, membrane, bending, lame_shear, lame_div);
+  int64_t V = X * Y * Z;
+  double w[3];
+  for (int d = 0; d < 3; ++d) w[d] = W.wd[d][0] * ONE_PLUS_TINY;
+  for (int64_t n = 0; n < N; ++n) {
+    const double *hn = h ? h + n * CC * V : NULL;
+    const double *gn = g + n * 3 * V;
+    double *on = out + n * 3 * V;
+    for (int64_t i = 0; i < V; ++i) {
+      double v0 = gn[i], v1 = gn[V + i], v2 = gn[2 * V + i];
+      if (!hn) {
+        on[i] = v0 / w[0]; on[V + i] = v1 / w[1]; on[2 * V + i] = v2 / w[2];
+      } else if (CC == 1) {
+        double h0 = hn[i] * ONE_PLUS_TINY;
+        on[i] = v0 / (h0 + w[0]); on[V + i] = v1 / (h0 + w[1]); on[2 * V + i] = v2 / (h0 + w[2]);
+      } else if (CC == 3) {
+        on[i] = v0 / (hn[i] * ONE_PLUS_TINY + w[0]);
+        on[V + i] = v1 / (hn[V + i] * ONE_PLUS_TINY + w[1]);
+        on[2 * V + i] = v2 / (hn[2 * V + i] * ONE_PLUS_TINY + w[2]);
+      } else {
+        double a = hn[i] * ONE_PLUS_TINY + w[0], b = hn[V + i] * ONE_PLUS_TINY + w[1];
+        double c = hn[2 * V + i] * ONE_PLUS_TINY + w[2];
+        double p = hn[3 * V + i], q = hn[4 * V + i], t = hn[5 * V + i];
+        double idt = 1.0 / (a * b * c - a * t * t - b * q * q - c * p * p + 2 * p * q * t);
+        on[i]         = idt * (v0 * (b * c - t * t) + v1 * (q * t - p * c) + v2 * (p * t - q * b));
+        on[V + i]     = idt * (v0 * (q * t - p * c) + v1 * (a * c - q * q) + v2 * (p * q - a * t));
+        on[2 * V + i] = idt * (v0 * (p * t - q * b) + v1 * (p * q - a * t) + v2 * (a * b - p * p));
+      }
+    }
+  }
+}
+
+/* Per-batch dot products (pcg_common.cpp:24-36). */
+static void batch_dot(const double *a, const double *b, int64_t N, int64_t n3, double *out)
+{
+  for (int64_t n = 0; n < N; ++n) {
+    double s = 0;
+    for (int64_t i = 0; i < n3; ++i) s += a[n * n3 + i] * b[n * n3 + i];
+    out[n] = s;
+  }
+}
+
+/* do_pcg (pcg_common.cpp:38-85); x in/out; returns the iterations done. */
+int or_pcg_grid(const double *h, const double *g, double *x, int CC,
+                int64_t N, int64_t X, int64_t Y, int64_t Z, const int *bound,
+                const double *voxel_size, double absolute, double membrane, double bending,
+                double lame_shear, double lame_div, int nb_iter, double tol)
+{
+  int64_t V = X * Y * Z, n3 = 3 * V, tot = N * n3;
+  double *r = malloc(sizeof(double) * tot), *z = malloc(sizeof(double) * tot);
+  double *p = malloc(sizeof(double) * tot);
+  double *rz = malloc(sizeof(double) * N), *rz0 = malloc(sizeof(double) * N);
+  double *pAp = malloc(sizeof(double) * N), *alpha = malloc(sizeof(double) * N);
+#define FWD(in, out) or_regulariser_grid(in, out, h, CC, N, X, Y, Z, bound, voxel_size, absolute, \
+                                         membrane, bending, lame_shear, lame_div)
+#define PRE(in, out) or_precond_grid(h, in, out, CC, N, X, Y, Z, voxel_size, absolute, membrane, \
+                                     bending, lame_shear, lame_div)
+  FWD(x, r);
+  for (int64_t i = 0; i < tot; ++i) r[i] = g[i] - r[i];
+  PRE(r, z);
+  batch_dot(r, z, N, n3, rz);
+  memcpy(p, z, sizeof(double) * tot);
+  int it = 0;
+  for (; it < nb_iter; ++it) {
+    FWD(p, z);
+    batch_dot(p, z, N, n3, pAp);
+    double obj = 0;
+    for (int64_t n = 0; n < N; ++n) {
+      alpha[n] = rz[n] / (pAp[n] > 1e-12 ? pAp[n] : 1e-12);
+      obj += alpha[n] * (alpha[n] * pAp[n] + 2 * rz[n]);
+    }
+    if (tol != 0 && obj < tol * (double)(N * V)) break;
+    for (int64_t n = 0; n < N; ++n)
+      for (int64_t i = n * n3; i < (n + 1) * n3; ++i) {
+        x[i] += alpha[n] * p[i];
+        r[i] -= alpha[n] * z[i];
+      }
+    PRE(r, z);
+    memcpy(rz0, rz, sizeof(double) * N);
+    batch_dot(r, z, N, n3, rz);
+    for (int64_t n = 0; n < N; ++n) {
+      double beta = rz[n] / (rz0[n] > 1e-12 ? rz0[n] : 1e-12);
+      for (int64_t i = n * n3; i < (n + 1) * n3; ++i) p[i] = z[i] + beta * p[i];
+    }
+  }
+#undef FWD
+#undef PRE
+  free(r); free(z); free(p); free(rz); free(rz0); free(pAp); free(alpha);
+  return it;
+}
+
 void or_fmg_grid(const double *h, const double *g, double *x, int CC,
                  int64_t N, int64_t X, int64_t Y, int64_t Z, const int *bound,
                  const double *voxel_size, double absolute, double membrane, double bending,
                  double lame_shear, double lame_div, int nb_cycles, int nb_iter, int max_levels,
-                 int quirk_fold, int quirk_zscale)
+                 int quirk_fold, int quirk_zscale, int use_cg)
 {
   int64_t sz[MAXL][3];
   double vs[MAXL][3];
@@ -426,8 +577,12 @@ void or_fmg_grid(const double *h, const double *g, double *x, int CC,
     scale_fields(Xs[L], N, 3, sz[L], f, 0, quirk_zscale);
   }
   int NL = L;
-#define RELAX(l) or_relax_grid(Gr[l], H[l], CC, Xs[l], N, sz[l][0], sz[l][1], sz[l][2], bound, vs[l], \
-                               absolute, membrane, bending, lame_shear, lame_div, nb_iter, quirk_fold)
+#define RELAX(l) do { \
+    if (use_cg) or_pcg_grid(H[l], Gr[l], Xs[l], CC, N, sz[l][0], sz[l][1], sz[l][2], bound, vs[l], \
+                            absolute, membrane, bending, lame_shear, lame_div, nb_iter, 0.0); \
+    else or_relax_grid(Gr[l], H[l], CC, Xs[l], N, sz[l][0], sz[l][1], sz[l][2], bound, vs[l], \
+                       absolute, membrane, bending, lame_shear, lame_div, nb_iter, quirk_fold); \
+  } while (0)
 #define FAC(l, f) do { for (int d_ = 0; d_ < 3; ++d_) f[d_] = (double)sz[l][d_] / (double)sz[(l) + 1][d_]; } while (0)
   RELAX(NL - 1);
   for (int base = NL - 2; base >= 0; --base) {

@@ -1,7 +1,8 @@
 """Write tests/golden/solver_golden.npz: vectors of the reference's own
 compiled CPU backend (oracle/_ref) for the displacement-field solver path:
 regulariser_grid, relax_grid, fmg_prolongation, fmg_restriction, fmg_grid
-(float64, small 3-D fields, batch 1).
+(Gauss-Seidel and PCG smoothers), precond_grid, pcg_grid (float64, small 3-D
+fields, batch 1).
 
     python tests/golden/make_solver_golden.py
 
@@ -99,9 +100,54 @@ def main():
                 out[key + "_h"] = h
                 out[key + "_sol"] = sol
                 f += 1
+    # precond_grid / pcg_grid (tol 0: fixed iterations; tol > 0: early stop)
+    c = 0
+    for shape in [(6, 5, 7)]:
+        for bname in ("dft", "dct2", "replicate", "dst2", "zero", "dct1"):
+            for ip, p in enumerate(PENALTIES):
+                CC = (0, 1, 3, 6)[(ip + c) % 4]
+                vs = [1.0, 1.3, 0.8]
+                g = rng.standard_normal((1, 3) + shape)
+                x = rng.standard_normal((1, 3) + shape) * 0.1
+                h = None
+                if CC:
+                    h = rng.uniform(0.5, 2, (1, CC) + shape)
+                    if CC == 6:
+                        h[:, 3:] = rng.uniform(-0.2, 0.2, (1, 3) + shape)
+                th = torch.from_numpy(h) if CC else E
+                B = R.bounds([bname])
+                key = f"c{c:04d}"
+                out[key + "_meta"] = np.array([BOUNDS.index(bname), CC], np.int64)
+                out[key + "_pen"] = np.array(p + tuple(vs))
+                out[key + "_g"] = g
+                out[key + "_x"] = x
+                if CC:
+                    out[key + "_h"] = h
+                out[key + "_precond"] = m.precond_grid(th, torch.from_numpy(g), E, E, *p, vs, B).numpy()
+                out[key + "_pcg"] = m.pcg_grid(th, torch.from_numpy(g), torch.from_numpy(x.copy()), E,
+                                               *p, vs, B, 5, 0.0).numpy()
+                out[key + "_pcgtol"] = m.pcg_grid(th, torch.from_numpy(g), torch.from_numpy(x.copy()), E,
+                                                  *p, vs, B, 50, 1e-3).numpy()
+                c += 1
+    # fmg_grid with the PCG smoother (use_cg)
+    for shape in [(12, 10, 9)]:
+        for bname in ("dft", "dct2"):
+            for p in PENALTIES[2:]:
+                g = rng.standard_normal((1, 3) + shape)
+                h = rng.uniform(0.5, 2, (1, 6) + shape)
+                h[:, 3:] = rng.uniform(-0.2, 0.2, (1, 3) + shape)
+                sol = m.fmg_grid(torch.from_numpy(h), torch.from_numpy(g), E, E, *p, [1., 1., 1.],
+                                 R.bounds([bname]), 2, 2, 16, True).numpy()
+                key = f"f{f:04d}"
+                out[key + "_meta"] = np.array([BOUNDS.index(bname)], np.int64)
+                out[key + "_pen"] = np.array(p)
+                out[key + "_g"] = g
+                out[key + "_h"] = h
+                out[key + "_solcg"] = sol
+                f += 1
     path = os.path.join(HERE, "solver_golden.npz")
     np.savez_compressed(path, **out)
-    print(path, k, t, f)
+    print(path, k, t, f, c)
 
 
 if __name__ == "__main__":

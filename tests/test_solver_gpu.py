@@ -74,14 +74,16 @@ def test_regulariser_vs_oracle(sv, dt, bound):
 @pytest.mark.parametrize("dt", [torch.float32, torch.float64])
 def test_relax_vs_oracle(sv, dt):
     rng = np.random.default_rng(2)
-    for shape in [(9, 8, 10), (40, 36, 34)]:       # single-block and multi-launch paths
+    # single-block and multi-launch paths; sizes with and without remainder
+    # colour classes on circulant axes
+    for shape in [(9, 8, 10), (12, 6, 18), (40, 36, 34), (30, 24, 36)]:
         for p in PENS:
             for bound in ("dft", "dct2"):
                 g, h = system(rng, shape)
                 x0 = rng.standard_normal(g.shape) * 0.1
                 x = T(x0, dt)
                 sv.relax_grid_cf(T(g, dt), x, T(h, dt), pen_vec(p), (1, 1, 1), bound, nb_iter=2)
-                ref = S.relax_grid(g, h, x0, bound=bound, nb_iter=2, **p)
+                ref = S.relax_grid(g, h, x0, bound=bound, nb_iter=2, product_order=True, **p)
                 tol = 2e-5 if dt == torch.float32 else 1e-9
                 assert rel(x, ref) <= tol, (shape, p, bound)
 
@@ -109,9 +111,22 @@ def test_fmg_vs_oracle(sv, dt):
             for bound in ("dft", "dct2"):
                 g, h = system(rng, shape)
                 x = sv.fmg_grid_cf(T(h, dt), T(g, dt), None, pen_vec(p), (1, 1, 1), bound, 2, 2)
-                ref = S.fmg_grid(h, g, bound=bound, nb_cycles=2, nb_iter=2, **p)
+                ref = S.fmg_grid(h, g, bound=bound, nb_cycles=2, nb_iter=2, product_order=True, **p)
                 tol = 1e-4 if dt == torch.float32 else 1e-9
                 assert rel(x, ref) <= tol, (shape, p, bound)
+
+
+def test_relax_deterministic_with_wrap(sv):
+    """Circulant axes whose length is not a multiple of the colour period:
+    repeated relaxations give bit-identical results (no races)."""
+    rng = np.random.default_rng(7)
+    g, h = system(rng, (41, 37, 40))
+    outs = []
+    for _ in range(2):
+        x = torch.zeros((1, 3, 41, 37, 40), dtype=torch.float32, device=DEV)
+        sv.relax_grid_cf(T(g, torch.float32), x, T(h, torch.float32), pen_vec(PENS[3]), (1, 1, 1), "dft", 3)
+        outs.append(x.cpu())
+    assert torch.equal(outs[0], outs[1])
 
 
 def test_solve_to_tolerance(sv):
@@ -138,9 +153,18 @@ def test_channel_last_api(sv):
     assert out.shape == gl.shape
     ref = np.moveaxis(S.regulariser_grid(g, h, **p), 1, -1)
     assert rel(out, ref) < 1e-10
-    sol = sv.fmg_grid(hl, gl, **p, nb_cycles=2, nb_iter=2)
-    ref = np.moveaxis(S.fmg_grid(h, g, nb_cycles=2, nb_iter=2, **p), 1, -1)
+    sol = sv.fmg_grid(hl, gl, **p, nb_cycles=2, nb_iter=2, solver="relax")
+    ref = np.moveaxis(S.fmg_grid(h, g, nb_cycles=2, nb_iter=2, product_order=True, **p), 1, -1)
     assert rel(sol, ref) < 1e-9
+    sol = sv.fmg_grid(hl, gl, **p, nb_cycles=2, nb_iter=2)          # reference default: PCG smoother
+    ref = np.moveaxis(S.fmg_grid(h, g, nb_cycles=2, nb_iter=2, use_cg=True, **p), 1, -1)
+    assert rel(sol, ref) < 1e-9
+    sol = sv.pcg_grid(gl, hessian=hl, **p, nb_iter=6, tol=0)
+    ref = np.moveaxis(S.pcg_grid(h, g, nb_iter=6, **p)[0], 1, -1)
+    assert rel(sol, ref) < 1e-9
+    sol = sv.precond_grid(gl, hessian=hl, **p)
+    ref = np.moveaxis(S.precond_grid(g, h, **p), 1, -1)
+    assert rel(sol, ref) < 1e-12
     # solve_grid_fmg without spatial penalties: closed form
     sol = sv.solve_grid_fmg(hl, gl, absolute=0.5)
     Hm = np.zeros(h.shape[:1] + h.shape[2:] + (3, 3))
@@ -151,3 +175,66 @@ def test_channel_last_api(sv):
     Hm += 0.5 * np.eye(3)
     ref = np.linalg.solve(Hm, np.moveaxis(g, 1, -1)[..., None])[..., 0]
     assert rel(sol, ref) < 1e-10
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_precond_vs_oracle(sv, dt):
+    rng = np.random.default_rng(8)
+    tol = 1e-6 if dt == torch.float32 else 1e-12
+    for shape in [(7, 6, 9), (16, 17, 5)]:
+        for p in PENS:
+            for CC in (0, 1, 3, 6):
+                for N in (1, 2):
+                    g, h = system(rng, shape, CC, N)
+                    vs = (1.0, 1.3, 0.8)
+                    out = sv.precond_grid_cf(T(h, dt), T(g, dt), pen_vec(p), vs)
+                    ref = S.precond_grid(g, h, voxel_size=vs, **p)
+                    assert rel(out, ref) <= tol, (shape, p, CC, N)
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_pcg_vs_oracle(sv, dt):
+    """Fixed-iteration PCG (tol 0, the FMG smoother) for all Hessian layouts,
+    bounds and batch sizes; sizes below and above one work-group per batch."""
+    rng = np.random.default_rng(9)
+    for shape in [(7, 6, 5), (20, 18, 22)]:
+        for p in PENS[1:]:
+            for bound in ("dft", "dct2", "zero"):
+                for CC, N in ((6, 1), (3, 2), (1, 1), (0, 2)):
+                    g, h = system(rng, shape, CC, N)
+                    if CC == 0 and bound == "dft" and p.get("absolute", 0) == 0:
+                        continue    # singular system
+                    x0 = rng.standard_normal(g.shape) * 0.1
+                    x = T(x0, dt)
+                    _, it = sv.pcg_grid_cf(T(h, dt), T(g, dt), x, pen_vec(p), (1, 1.2, 0.9), bound, nb_iter=4)
+                    assert it == 4
+                    ref, _ = S.pcg_grid(h, g, x0, voxel_size=(1, 1.2, 0.9), bound=bound, nb_iter=4, **p)
+                    tol = 1e-4 if dt == torch.float32 else 1e-9
+                    assert rel(x, ref) <= tol, (shape, p, bound, CC, N)
+
+
+def test_pcg_early_stop_vs_oracle(sv):
+    """tol > 0: same stopping iteration and iterate as the oracle (FP64)."""
+    rng = np.random.default_rng(10)
+    for shape in [(9, 8, 10), (24, 20, 16)]:
+        for p in PENS[1:4]:
+            for bound in ("dct2", "replicate"):
+                g, h = system(rng, shape, 6, 1)
+                x, it = sv.pcg_grid_cf(T(h, torch.float64), T(g, torch.float64), None, pen_vec(p), (1, 1, 1),
+                                       bound, nb_iter=60, tol=1e-3)
+                ref, it_ref = S.pcg_grid(h, g, bound=bound, nb_iter=60, tol=1e-3, **p)
+                assert it == it_ref, (shape, p, bound)
+                assert rel(x, ref) <= 1e-9, (shape, p, bound)
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_fmg_cg_vs_oracle(sv, dt):
+    rng = np.random.default_rng(11)
+    for shape in [(12, 10, 9), (24, 20, 18)]:
+        for p in PENS[2:]:
+            for bound in ("dft", "dct2"):
+                g, h = system(rng, shape)
+                x = sv.fmg_grid_cf(T(h, dt), T(g, dt), None, pen_vec(p), (1, 1, 1), bound, 2, 2, use_cg=True)
+                ref = S.fmg_grid(h, g, bound=bound, nb_cycles=2, nb_iter=2, use_cg=True, **p)
+                tol = 2e-4 if dt == torch.float32 else 1e-9
+                assert rel(x, ref) <= tol, (shape, p, bound)

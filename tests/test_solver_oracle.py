@@ -1,5 +1,5 @@
 """CPU tests of the solver oracle (FP64 restatement of regulariser_grid,
-relax_grid, FMG transfers and fmg_grid).
+relax_grid, FMG transfers, precond_grid, pcg_grid and fmg_grid).
 
 Pinned by the reference's own vectors (tests/golden/solver_golden.npz, written
 by tests/golden/make_solver_golden.py from oracle/_ref) and by closed forms:
@@ -58,14 +58,98 @@ def test_transfers_golden(golden):
 
 
 def test_fmg_golden(golden):
+    n = 0
     for key, c in golden.items():
         if not key.startswith("f"):
             continue
         b = BOUNDS[int(c["meta"][0])]
         a, m, be, mu, la = [float(v) for v in c["pen"]]
-        out = S.fmg_grid(c["h"], c["g"], absolute=a, membrane=m, bending=be, lame=(mu, la), bound=b,
-                         nb_cycles=2, nb_iter=2, quirk_fold=True, quirk_zscale=True)
-        assert rel(out, c["sol"]) < 1e-10, key
+        kw = dict(absolute=a, membrane=m, bending=be, lame=(mu, la), bound=b, nb_cycles=2, nb_iter=2)
+        if "sol" in c:
+            out = S.fmg_grid(c["h"], c["g"], quirk_fold=True, quirk_zscale=True, **kw)
+            assert rel(out, c["sol"]) < 1e-10, key
+        else:
+            out = S.fmg_grid(c["h"], c["g"], quirk_zscale=True, use_cg=True, **kw)
+            assert rel(out, c["solcg"]) < 1e-10, key
+        n += 1
+    assert n == 12
+
+
+def test_precond_and_pcg_golden(golden):
+    n = 0
+    for key, c in golden.items():
+        if not key.startswith("c"):
+            continue
+        b, CC = [int(v) for v in c["meta"]]
+        a, m, be, mu, la, *vs = [float(v) for v in c["pen"]]
+        kw = dict(absolute=a, membrane=m, bending=be, lame=(mu, la), voxel_size=vs)
+        h = c.get("h")
+        assert rel(S.precond_grid(c["g"], h, **kw), c["precond"]) < 1e-13, key
+        x, it = S.pcg_grid(h, c["g"], c["x"], bound=BOUNDS[b], nb_iter=5, tol=0.0, **kw)
+        assert it == 5 and rel(x, c["pcg"]) < 1e-11, key
+        if BOUNDS[b] == "dft" and h is None:
+            # circulant L with no Hessian: (near-)singular system, the early-stop
+            # iterate is rounding-dominated on both sides
+            n += 1
+            continue
+        x, it = S.pcg_grid(h, c["g"], c["x"], bound=BOUNDS[b], nb_iter=50, tol=1e-3, **kw)
+        assert rel(x, c["pcgtol"]) < 1e-11, key
+        # the stopping iteration matters: one more or one fewer is far off
+        x1, _ = S.pcg_grid(h, c["g"], c["x"], bound=BOUNDS[b], nb_iter=it + 1, tol=0.0, **kw)
+        assert it == 1 or rel(x1, c["pcgtol"]) > 1e-6, key
+        n += 1
+    assert n == 30
+
+
+def test_precond_is_block_diagonal_inverse():
+    """(H + diag L) applied voxel-wise to precond(g) gives g back (up to the
+    1e-6 diagonal inflation), for every Hessian layout."""
+    rng = np.random.default_rng(11)
+    shape = (4, 5, 3)
+    kw = dict(absolute=0.2, membrane=0.1, bending=0.05, lame=(0.3, 0.4), voxel_size=(1.0, 1.2, 0.9))
+    g = rng.standard_normal((2, 3) + shape)
+    from oracle.solver import _setup  # noqa: F401  (library load)
+    vx = 1 / np.array(kw["voxel_size"]) ** 2
+    l0, l1, l2, mu, la = 0.2, 0.1, 0.05, 0.3, 0.4
+    w000 = l2 * (6 * (vx ** 2).sum() + 8 * (vx[0] * vx[1] + vx[0] * vx[2] + vx[1] * vx[2])) + l1 * 2 * vx.sum() + l0
+    diag = np.array([2 * mu * (vx.sum() + vx[d]) / vx[d] + 2 * la + w000 / vx[d] for d in range(3)])
+    for CC in (0, 1, 3, 6):
+        h = None
+        Hm = np.zeros((2,) + shape + (3, 3))
+        if CC:
+            h = rng.uniform(0.5, 2, (2, CC) + shape)
+            if CC == 6:
+                h[:, 3:] = rng.uniform(-0.2, 0.2, (2, 3) + shape)
+            hh = np.moveaxis(h, 1, -1)
+            pairs = {1: [(0, 0, 0), (0, 1, 1), (0, 2, 2)], 3: [(0, 0, 0), (1, 1, 1), (2, 2, 2)],
+                     6: [(0, 0, 0), (1, 1, 1), (2, 2, 2), (3, 0, 1), (4, 0, 2), (5, 1, 2)]}[CC]
+            for k, i, j in pairs:
+                Hm[..., i, j] = hh[..., k]
+                Hm[..., j, i] = hh[..., k]
+        A = Hm + np.diag(diag)
+        z = S.precond_grid(g, h, **kw)
+        back = np.einsum("...ij,...j->...i", A, np.moveaxis(z, 1, -1))
+        assert rel(back, np.moveaxis(g, 1, -1)) < 1e-5
+
+
+def test_pcg_converges_to_cg():
+    """Block-Jacobi PCG reaches the exact (CG to 1e-12) solution, with the
+    energy norm of the error decreasing at every iteration."""
+    rng = np.random.default_rng(12)
+    shape = (10, 9, 8)
+    g = rng.standard_normal((1, 3) + shape)
+    h = rng.uniform(0.5, 2, (1, 6) + shape)
+    h[:, 3:] = rng.uniform(-0.2, 0.2, (1, 3) + shape)
+    kw = dict(absolute=1e-4, membrane=1e-3, bending=0.2, lame=(0.05, 0.2), bound="dct2")
+    xs, _ = S.cg_grid(h, g, tol=1e-12, **kw)
+    prev = np.inf
+    for it in (1, 2, 4, 8, 16):
+        x, _ = S.pcg_grid(h, g, nb_iter=it, **kw)
+        e = np.sum((x - xs) * S.regulariser_grid(x - xs, h, **kw))
+        assert e < prev
+        prev = e
+    x, _ = S.pcg_grid(h, g, nb_iter=400, **kw)
+    assert rel(x, xs) < 1e-6
 
 
 @pytest.mark.parametrize("pen", [dict(absolute=0.3), dict(membrane=0.7, absolute=0.01),
