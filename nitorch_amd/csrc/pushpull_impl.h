@@ -416,10 +416,10 @@ template <> bool dispatch_order<7>(const PPArgs<float> &, int64_t, hipStream_t);
 #define NI_DEFINE_DISPATCH_ORDER(ORD)                                            \
   template <> bool dispatch_order<ORD>(const PPArgs<float> &a, int64_t total, hipStream_t st) { \
     switch (a.flags) {                                                          \
-      case F_PULL: launch_fast<float, int32_t, ORD, F_PULL>(a, total, st); return true; \
+      case F_PULL: launch_tile<ORD, TM_PULL>(a, st); return true; \
       case F_SGRAD: launch_fast<float, int32_t, ORD, F_SGRAD>(a, total, st); return true; \
-      case F_PUSH | F_TRGT: launch_fast<float, int32_t, ORD, F_PUSH | F_TRGT>(a, total, st); return true; \
-      case F_COUNT: launch_fast<float, int32_t, ORD, F_COUNT>(a, total, st); return true; \
+      case F_PUSH | F_TRGT: launch_tile<ORD, TM_PUSH>(a, st); return true; \
+      case F_COUNT: launch_tile<ORD, TM_COUNT>(a, st); return true; \
       case F_PUSH | F_GRAD | F_TRGT: launch_fast<float, int32_t, ORD, F_PUSH | F_GRAD | F_TRGT>(a, total, st); return true; \
       case F_GRAD | F_TRGT: launch_fast<float, int32_t, ORD, F_GRAD | F_TRGT>(a, total, st); return true; \
       case F_PULL | F_GRAD | F_TRGT: launch_fast<float, int32_t, ORD, F_PULL | F_GRAD | F_TRGT>(a, total, st); return true; \

@@ -1,6 +1,6 @@
 // Order-1 specialisations of the 3-D float push/pull kernels (one
 // translation unit per order so that the library builds in parallel).
-#include "pushpull_impl.h"
+#include "pushpull_tile.h"
 
 namespace ni_amd {
 NI_DEFINE_DISPATCH_ORDER(1)

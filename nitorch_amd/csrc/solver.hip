@@ -46,7 +46,7 @@
 namespace ni_amd {
 
 #define SV_BLOCK 256
-#define SV_SMALL 16384          // levels up to this many voxels: single-block relax
+#define SV_SMALL 4096           // levels up to this many voxels (x batch): single-block relax
 
 struct StencilW {
   double vx[3];        // 1 / voxel_size^2
@@ -316,16 +316,17 @@ __global__ __launch_bounds__(SV_BLOCK) void relax_colour(T *__restrict__ x, cons
 template <typename T>
 __global__ __launch_bounds__(1024) void relax_small(T *__restrict__ x, const T *__restrict__ g, const T *__restrict__ h,
                                                     int CC, Geo G, StencilT<T> S, int N, ColourPlan P, int nb_iter) {
+  // 32-bit index arithmetic: levels relaxed here hold at most SV_SMALL voxels
   const int V = G.X * G.Y * G.Z;
   for (int it = 0; it < nb_iter; ++it) {
     if (P.redblack) {
       for (int rb = 0; rb < 2; ++rb) {
         const int nzh = (G.Z + 1) / 2;
-        const long long per = (long long)G.X * G.Y * nzh;
-        for (long long k = threadIdx.x; k < per * N; k += blockDim.x) {
-          const int n = (int)(k / per), j = (int)(k % per);
+        const int per = G.X * G.Y * nzh;
+        for (int k = threadIdx.x; k < per * N; k += blockDim.x) {
+          const int n = k / per, j = k - n * per;
           const int row = j / nzh, y = row % G.Y, xx = row / G.Y;
-          const int z = (j % nzh) * 2 + ((xx + y + rb) & 1);
+          const int z = (j - row * nzh) * 2 + ((xx + y + rb) & 1);
           if (z < G.Z)
             relax_voxel<T>(x + (size_t)n * 3 * V, g + (size_t)n * 3 * V, h ? h + (size_t)n * CC * V : nullptr,
                            CC, V, G, S, xx, y, z);
@@ -338,12 +339,13 @@ __global__ __launch_bounds__(1024) void relax_small(T *__restrict__ x, const T *
       for (int b = 0; b < P.ax[1].n; ++b)
         for (int c = 0; c < P.ax[2].n; ++c) {
           const int nx = P.ax[0].count[a], ny = P.ax[1].count[b], nz = P.ax[2].count[c];
-          const long long per = (long long)nx * ny * nz;
-          for (long long k = threadIdx.x; k < per * N; k += blockDim.x) {
-            const int n = (int)(k / per), j = (int)(k % per);
-            const int z = P.ax[2].start[c] + P.ax[2].step[c] * (j % nz);
-            const int y = P.ax[1].start[b] + P.ax[1].step[b] * ((j / nz) % ny);
-            const int xx = P.ax[0].start[a] + P.ax[0].step[a] * (j / (nz * ny));
+          const int nyz = ny * nz, per = nx * nyz;
+          for (int k = threadIdx.x; k < per * N; k += blockDim.x) {
+            const int n = k / per, j = k - n * per;
+            const int jx = j / nyz, r = j - jx * nyz, jy = r / nz, jz = r - jy * nz;
+            const int z = P.ax[2].start[c] + P.ax[2].step[c] * jz;
+            const int y = P.ax[1].start[b] + P.ax[1].step[b] * jy;
+            const int xx = P.ax[0].start[a] + P.ax[0].step[a] * jx;
             relax_voxel<T>(x + (size_t)n * 3 * V, g + (size_t)n * 3 * V, h ? h + (size_t)n * CC * V : nullptr,
                            CC, V, G, S, xx, y, z);
           }
@@ -356,8 +358,10 @@ __global__ __launch_bounds__(1024) void relax_small(T *__restrict__ x, const T *
 // Preconditioned conjugate gradient (nitorch/_C/src/impl/pcg_common.cpp:38-85)
 // with the block-Jacobi preconditioner z = (H (1 + 1e-6) + diag(L)) \ r
 // (precond_grid_common.cpp:266-298, :620-660).  Scalars (r.z, p.Ap) stay on
-// the device, double precision, one slot per batch element and two
-// generations (ping-pong), so an iteration needs no host synchronisation.
+// the device, double precision, one slot per batch element; r.z lives in a
+// ring of three generations so that each iteration's last kernel can clear
+// the slot the next one accumulates into, and an iteration needs no host
+// synchronisation.
 // --------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ void block_jacobi(const T *__restrict__ hn, int CC, int V, int i, const StencilT<T> &S,
