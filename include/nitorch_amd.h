@@ -185,7 +185,8 @@ int ni_fmg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hess
                 int32_t use_cg, int32_t extra_vcycles, void *workspace, int64_t workspace_bytes, void *stream);
 
 /* Time-to-tolerance solve (no reference counterpart: the benchmark's "GN step
- * to 1e-4 relative residual"): ni_fmg_grid's full multigrid, then V-cycles on
+ * to 1e-4 relative residual"): ni_fmg_grid's full multigrid (skipped when
+ * nb_cycles <= 0: `solution` is then the starting iterate), then V-cycles on
  * the finest level until ||g - (H+L)x|| / ||g|| <= tol or max_vcycles.
  * `norms` is a device buffer of 2 doubles; the call synchronises `stream`
  * once per V-cycle to read the residual.  Reports the final relative
@@ -217,6 +218,51 @@ int ni_pcg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hess
 int ni_precond_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
                     const void *gradient, void *output, const int32_t *bound, const double *voxel_size,
                     const double *penalty, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Learned preconditioner (3-D U-Net on BF16 matrix-core kernels).  No
+ * counterpart in the reference: used by the learned-PCG Gauss-Newton solve.
+ * Activations are channels-last [N, *shape, C] bf16 with C a multiple of 16;
+ * weights of a CI -> CO layer are [CO][KP] bf16, K = tap * CI + ci with
+ * tap = 9 dx + 3 dy + dz (offsets -1..1 <-> 0..2), KP = 27 CI rounded up to
+ * 32 (zero-padded).  Zero padding of 1, stride 1, FP32 accumulation.
+ * ---------------------------------------------------------------------- */
+
+/* y = act(conv3x3x3(x, w) + bias): act 0 none / 1 leaky ReLU (slope 0.1);
+ * out_f32 = 1 writes fp32 (linear head; out_channels = 3 with CO = 16 keeps
+ * the first 3 channels), else bf16 with out_channels = CO.  bias may be NULL.
+ * Supported CI -> CO: 16->16, 16->32, 32->32, 32->64, 64->64, 96->32, 48->16
+ * and (backward data) 32->16, 64->32, 32->96, 16->48. */
+int ni_unet_conv3_fwd(int64_t N, const int64_t *shape, int32_t CI, int32_t CO, const void *x, const void *w,
+                      const float *bias, void *y, int32_t act, int32_t out_f32, int32_t out_channels, void *stream);
+
+/* dw[CO][KP] += sum over positions of dy (x) shifted x (fp32 accumulators,
+ * caller zeroes them); db[CO] += sum of dy if db is not NULL. */
+int ni_unet_conv3_bwd_weight(int64_t N, const int64_t *shape, int32_t CI, int32_t CO, const void *x, const void *dy,
+                             float *dw, float *db, void *stream);
+
+/* out = dy * (y > 0 ? 1 : 0.1), n bf16 elements. */
+int ni_unet_lrelu_bwd(int64_t n, const void *dy, const void *y, void *out, void *stream);
+
+/* 2x2x2 average pooling of [N, *shape, C] (backward: in = gradient of the
+ * pooled tensor, out = gradient at full resolution). */
+int ni_unet_avgpool2(int32_t backward, int64_t N, const int64_t *shape, int32_t C, const void *in, void *out,
+                     void *stream);
+
+/* out [N, *shape, C1 + C2] = concat(nearest 2x upsampling of a
+ * [N, *shape/2, C1], skip [N, *shape, C2]); backward: out is the incoming
+ * gradient, a and skip receive theirs. */
+int ni_unet_up2_concat(int32_t backward, int64_t N, const int64_t *shape, int32_t C1, int32_t C2, const void *a,
+                       const void *skip, void *out, void *stream);
+
+/* out [N, V, 16] bf16 = (sg * g, sh * h, 0...) from channel-first fp32
+ * g [N, 3, V] and h [N, CC, V] (h may be NULL). */
+int ni_unet_pack_input(int64_t N, int64_t V, const float *g, const float *h, int32_t CC, void *out, float sg,
+                       float sh, void *stream);
+
+/* x [N, 3, V] (channel-first fp32) = scale * y [N, V, 3] (+ x if accumulate). */
+int ni_unet_unpack_output(int64_t N, int64_t V, const float *y, float *x, float scale, int32_t accumulate,
+                          void *stream);
 
 #ifdef __cplusplus
 }

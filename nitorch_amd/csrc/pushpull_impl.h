@@ -413,17 +413,23 @@ template <> bool dispatch_order<5>(const PPArgs<float> &, int64_t, hipStream_t);
 template <> bool dispatch_order<6>(const PPArgs<float> &, int64_t, hipStream_t);
 template <> bool dispatch_order<7>(const PPArgs<float> &, int64_t, hipStream_t);
 
+// Gathers, scatters and their combinations run on the LDS-tiled kernels
+// (pushpull_tile.h); modes with a spatial-gradient target (trgt_K = dim)
+// keep the direct per-sample kernel.
 #define NI_DEFINE_DISPATCH_ORDER(ORD)                                            \
   template <> bool dispatch_order<ORD>(const PPArgs<float> &a, int64_t total, hipStream_t st) { \
+    (void)total;                                                                \
     switch (a.flags) {                                                          \
-      case F_PULL: launch_tile<ORD, TM_PULL>(a, st); return true; \
-      case F_SGRAD: launch_fast<float, int32_t, ORD, F_SGRAD>(a, total, st); return true; \
-      case F_PUSH | F_TRGT: launch_tile<ORD, TM_PUSH>(a, st); return true; \
-      case F_COUNT: launch_tile<ORD, TM_COUNT>(a, st); return true; \
-      case F_PUSH | F_GRAD | F_TRGT: launch_fast<float, int32_t, ORD, F_PUSH | F_GRAD | F_TRGT>(a, total, st); return true; \
-      case F_GRAD | F_TRGT: launch_fast<float, int32_t, ORD, F_GRAD | F_TRGT>(a, total, st); return true; \
-      case F_PULL | F_GRAD | F_TRGT: launch_fast<float, int32_t, ORD, F_PULL | F_GRAD | F_TRGT>(a, total, st); return true; \
-      case F_GRAD: launch_fast<float, int32_t, ORD, F_GRAD>(a, total, st); return true; \
+      case F_PULL: launch_tile<ORD, TM_PULL>(a, st); return true;               \
+      case F_SGRAD: launch_tile<ORD, TM_SGRAD>(a, st); return true;             \
+      case F_PUSH | F_TRGT: launch_tile<ORD, TM_PUSH>(a, st); return true;      \
+      case F_COUNT: launch_tile<ORD, TM_COUNT>(a, st); return true;             \
+      case F_PUSH | F_GRAD | F_TRGT:                                            \
+        launch_tile<ORD, TM_PUSH>(a, st); launch_tile<ORD, TM_GGRAD>(a, st); return true; \
+      case F_GRAD | F_TRGT: launch_tile<ORD, TM_GGRAD>(a, st); return true;     \
+      case F_PULL | F_GRAD | F_TRGT:                                            \
+        launch_tile<ORD, TM_PULL>(a, st); launch_tile<ORD, TM_GGRAD>(a, st); return true; \
+      case F_GRAD: launch_tile<ORD, TM_GGRAD1>(a, st); return true;             \
       default: return false;                                                    \
     }                                                                           \
   }

@@ -6,9 +6,14 @@
 //  * pull  : stages the box of the source -- folded by the boundary
 //            condition and signed, so the taps need no folding -- in LDS
 //            with coalesced loads, then gathers every tap from LDS;
-//  * push / count : accumulates the tile's splats into an LDS box with LDS
-//            atomics, then adds the box to the output once, folded and
-//            signed, with one global atomic per non-zero box element.
+//  * push / count : accumulates the tile's splats into an LDS box with
+//            integer LDS atomics on a fixed-point scale chosen per tile and
+//            channel (2^k with  n_in * max|value| * 2^k < 2^30, so no sum can
+//            overflow; LDS float atomics are far slower than integer ones on
+//            gfx950), then adds the box to the output once, folded and
+//            signed, with one FP32 global atomic per non-zero box element.
+//            Rounding: <= 2^-(21 - log2 n_in) * max|value| per splat, and the
+//            in-tile sum is exact and order-independent.
 // The box has fixed dimensions per order (every tap offset is then an
 // immediate of the LDS instruction); channels are staged NCC at a time.
 // A tile whose box does not fit (scattered or far out-of-range
@@ -47,24 +52,44 @@ struct TileBox {
   static constexpr int MAX_NCC = ORD <= 3 ? 4 : 2;   // channels staged at once (LDS budget)
 };
 
-enum TileMode { TM_PULL = 0, TM_PUSH = 1, TM_COUNT = 2 };
+// Gather modes stage the source box; scatter modes accumulate one.
+//   TM_PULL  : out[c]      = sum_t w(t) src(t, c)                   (F_PULL)
+//   TM_SGRAD : out[c][d]   = sum_t dw_d(t) src(t, c)                (F_SGRAD)
+//   TM_GGRAD : gout[d]     = sum_t dw_d(t) sum_c src(t, c) trgt(c)  (F_GRAD | F_TRGT,
+//              or F_GRAD with a single channel and no target: trgt = 1)
+//   TM_PUSH  : out(t, c)  += w(t) trgt(c)                           (F_PUSH | F_TRGT)
+//   TM_COUNT : out(t)     += w(t)                                   (F_COUNT)
+enum TileMode { TM_PULL = 0, TM_PUSH = 1, TM_COUNT = 2, TM_SGRAD = 3, TM_GGRAD = 4, TM_GGRAD1 = 5 };
 
-// First tap index and the ORD + 1 weights of coordinate x (as pp_sample).
-template <int ORD>
-__device__ __forceinline__ int tl_taps(float x, float (&w)[ORD + 1]) {
+template <int MODE>
+struct TileMeta {
+  static constexpr bool gather = MODE == TM_PULL || MODE == TM_SGRAD || MODE == TM_GGRAD || MODE == TM_GGRAD1;
+  static constexpr bool deriv = MODE == TM_SGRAD || MODE == TM_GGRAD || MODE == TM_GGRAD1;
+};
+
+// First tap index and the ORD + 1 weights (and, when DERIV, first
+// derivatives) of coordinate x, as pp_sample computes them.
+template <int ORD, bool DERIV = false>
+__device__ __forceinline__ int tl_taps(float x, float (&w)[ORD + 1], float (&g)[ORD + 1]) {
   if constexpr (ORD == 0) {
     w[0] = 1.f;
+    g[0] = 0.f;
     return (int)roundf(x);
   } else if constexpr (ORD == 1) {
     const int j0 = (int)floorf(x);
     const float d1 = x - (float)j0;
     w[0] = 1.f - d1;
     w[1] = d1;
+    g[0] = -1.f;
+    g[1] = 1.f;
     return j0;
   } else {
     const int lo = (int)floorf(x - first_tap_offset<float>(ORD));
 #pragma unroll
-    for (int t = 0; t <= ORD; ++t) w[t] = spl_w<float, ORD>(t, x - (float)(lo + t));
+    for (int t = 0; t <= ORD; ++t) {
+      w[t] = spl_w<float, ORD>(t, x - (float)(lo + t));
+      if (DERIV) g[t] = spl_g<float, ORD>(t, x - (float)(lo + t));
+    }
     return lo;
   }
 }
@@ -82,7 +107,9 @@ __device__ __forceinline__ int wave_max(int v) {
 
 template <int MODE>
 struct TileFlags {
-  static constexpr int value = MODE == TM_PULL ? F_PULL : MODE == TM_PUSH ? (F_PUSH | F_TRGT) : F_COUNT;
+  static constexpr int value = MODE == TM_PULL ? F_PULL : MODE == TM_PUSH ? (F_PUSH | F_TRGT)
+                               : MODE == TM_COUNT ? F_COUNT : MODE == TM_SGRAD ? F_SGRAD
+                               : MODE == TM_GGRAD ? (F_GRAD | F_TRGT) : F_GRAD;
 };
 
 // NCC: channels per chunk (compile time, so that the channel loop of a tap
@@ -108,8 +135,11 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
   // ---- samples of this thread: coordinates, in-bound flag, taps ----
   int px[TL_PER_THREAD], py[TL_PER_THREAD], pz[TL_PER_THREAD];
   bool valid[TL_PER_THREAD], inb[TL_PER_THREAD];
+  constexpr bool GATHER = TileMeta<MODE>::gather, DERIV = TileMeta<MODE>::deriv;
+  constexpr int NG = DERIV ? NT : 1;
   int lo[TL_PER_THREAD][3];
   float w[TL_PER_THREAD][3][NT];
+  float gw[TL_PER_THREAD][3][NG];   // derivative weights (gradient modes)
   int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
   const float tol = (a.extrap & 2) ? 0.55f : 0.05f;
 #pragma unroll
@@ -132,7 +162,10 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
       if (ok) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          lo[k][d] = tl_taps<ORD>(xs[d], w[k][d]);
+          float gtmp[NT];
+          lo[k][d] = tl_taps<ORD, DERIV>(xs[d], w[k][d], gtmp);
+#pragma unroll
+          for (int t = 0; t < NG; ++t) gw[k][d][t] = gtmp[t];
           mn[d] = min(mn[d], lo[k][d]);
           mx[d] = max(mx[d], lo[k][d]);
         }
@@ -165,15 +198,23 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
     }
   }
 
-  const int oC = (int)a.out_s[1];
+  const int oC = (int)a.out_s[1], oK = (int)a.out_s[5];
   if (mn[0] > mx[0]) {   // no in-bound sample in this tile
-    if (MODE == TM_PULL) {
-      const int C = (int)a.C;
+    if (MODE == TM_PULL || MODE == TM_SGRAD) {
+      const int C = (int)a.C, nk = MODE == TM_SGRAD ? 3 : 1;
 #pragma unroll
       for (int k = 0; k < TL_PER_THREAD; ++k)
         if (valid[k]) {
           float *op = a.out + (int)(n * a.out_s[0] + px[k] * a.out_s[2] + py[k] * a.out_s[3] + pz[k] * a.out_s[4]);
-          for (int c = 0; c < C; ++c) op[c * oC] = 0.f;
+          for (int c = 0; c < C; ++c)
+            for (int d = 0; d < nk; ++d) op[c * oC + d * oK] = 0.f;
+        }
+    } else if (MODE == TM_GGRAD || MODE == TM_GGRAD1) {
+#pragma unroll
+      for (int k = 0; k < TL_PER_THREAD; ++k)
+        if (valid[k]) {
+          float *gp = a.gout + (int)(n * a.gout_s[0] + px[k] * a.gout_s[1] + py[k] * a.gout_s[2] + pz[k] * a.gout_s[3]);
+          for (int d = 0; d < 3; ++d) gp[d * a.gout_s[4]] = 0.f;
         }
     }
     return;
@@ -193,6 +234,14 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
     off[k] = inb[k] ? ((lo[k][0] - mn[0]) * DY + (lo[k][1] - mn[1])) * DZ + (lo[k][2] - mn[2]) : 0;
 
   const int C = MODE == TM_COUNT ? 1 : (int)a.C;
+  // scatter: number of in-bound samples (bounds every fixed-point box sum)
+  int n_in = 0;
+  if (!GATHER) {
+#pragma unroll
+    for (int k = 0; k < TL_PER_THREAD; ++k) n_in += __syncthreads_count(inb[k]);
+  }
+  __shared__ float s_red[NI_BLOCK / 64][NCC];
+  __shared__ float s_scale[NCC], s_inv[NCC];
 
   // folded offsets (in elements of the staged / scattered tensor) and signs
   // of the box positions along each axis: fold() runs once per position
@@ -200,7 +249,7 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
   __shared__ int fofs[NF];
   __shared__ float fsg[NF];
   {
-    const int64_t *st = MODE == TM_PULL ? a.src_s : a.out_s;
+    const int64_t *st = GATHER ? a.src_s : a.out_s;
     const int st0 = (int)st[2], st1 = (int)st[3], st2 = (int)st[4];
     for (int i = threadIdx.x; i < NF; i += NI_BLOCK) {
       const int d = i < B::DX ? 0 : i < B::DX + DY ? 1 : 2;
@@ -217,6 +266,10 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
   }
   __syncthreads();
 
+  float og[TL_PER_THREAD][3];   // grid gradient (TM_GGRAD*), summed over chunks
+#pragma unroll
+  for (int k = 0; k < TL_PER_THREAD; ++k) og[k][0] = og[k][1] = og[k][2] = 0.f;
+
   for (int c0 = 0; c0 < C; c0 += NCC) {
     const int nc = (C - c0) < NCC ? (C - c0) : NCC;
     if (c0) __syncthreads();   // previous chunk fully consumed
@@ -225,30 +278,91 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
 #pragma unroll
     for (int k = 0; k < TL_PER_THREAD; ++k)
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
+      for (int d = 0; d < 3; ++d) {
 #pragma unroll
         for (int t = 0; t < NT; ++t) asm volatile("" : "+v"(w[k][d][t]));
+#pragma unroll
+        for (int t = 0; t < NG; ++t) asm volatile("" : "+v"(gw[k][d][t]));
+      }
     const int nbox = nc * VOL;
 
-    if (MODE == TM_PULL) {
+    if (GATHER) {
       // ---- stage the folded, signed source box ----
       const float *src = a.src + (int)(n * a.src_s[0] + c0 * a.src_s[1]);
       const int sC = (int)a.src_s[1];
-      for (int i = threadIdx.x; i < nbox; i += NI_BLOCK) {
-        const int c = i / VOL;
-        const int rr = i - c * VOL;
-        const int bx = rr / DYZ;
-        const int rem = rr - bx * DYZ;
-        const int by = rem / DZ;
-        const int bz = rem - by * DZ;
-        if (bx >= ext[0] || by >= ext[1] || bz >= ext[2]) continue;   // never read
+      // only the ext[0] x ext[1] x ext[2] corner of the box is ever read:
+      // enumerate it directly (floor divisions by float reciprocals, exact
+      // after the +-1 fix-ups for indices below 2^22)
+      const int E2 = ext[2], E12 = ext[1] * ext[2], E = ext[0] * E12;
+      const float rE = 1.f / (float)E, r12 = 1.f / (float)E12, r2 = 1.f / (float)E2;
+      const int nfill = nc * E;
+#pragma unroll 4
+      for (int i = threadIdx.x; i < nfill; i += NI_BLOCK) {
+        int c = (int)((float)i * rE);
+        c -= c * E > i;
+        c += (c + 1) * E <= i;
+        const int rr = i - c * E;
+        int bx = (int)((float)rr * r12);
+        bx -= bx * E12 > rr;
+        bx += (bx + 1) * E12 <= rr;
+        const int rem = rr - bx * E12;
+        int by = (int)((float)rem * r2);
+        by -= by * E2 > rem;
+        by += (by + 1) * E2 <= rem;
+        const int bz = rem - by * E2;
         const float sg = fsg[bx] * fsg[B::DX + by] * fsg[B::DX + DY + bz];
         float v = 0.f;
         if (sg != 0.f) v = src[c * sC + fofs[bx] + fofs[B::DX + by] + fofs[B::DX + DY + bz]] * sg;
-        box[i] = v;
+        box[c * VOL + (bx * DY + by) * DZ + bz] = v;
       }
     } else {
       for (int i = threadIdx.x; i < NCC * VOL; i += NI_BLOCK) box[i] = 0.f;
+    }
+    // scatter: values of this chunk and the per-channel fixed-point scales
+    float tv[TL_PER_THREAD][NCC];
+    if (!GATHER) {
+      float mloc[NCC];
+#pragma unroll
+      for (int c = 0; c < NCC; ++c) mloc[c] = 0.f;
+#pragma unroll
+      for (int k = 0; k < TL_PER_THREAD; ++k) {
+#pragma unroll
+        for (int c = 0; c < NCC; ++c) tv[k][c] = 0.f;
+        if (!inb[k]) continue;
+        if (MODE == TM_PUSH) {
+          const float *tp = a.trgt + (int)(n * a.trgt_s[0] + (int64_t)c0 * a.trgt_s[1] + px[k] * a.trgt_s[2] +
+                                           py[k] * a.trgt_s[3] + pz[k] * a.trgt_s[4]);
+          const int tC = (int)a.trgt_s[1];
+#pragma unroll
+          for (int c = 0; c < NCC; ++c) tv[k][c] = c < nc ? tp[c * tC] : 0.f;
+        } else {
+          tv[k][0] = 1.f;
+        }
+#pragma unroll
+        for (int c = 0; c < NCC; ++c) mloc[c] = fmaxf(mloc[c], fabsf(tv[k][c]));
+      }
+#pragma unroll
+      for (int c = 0; c < NCC; ++c) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) mloc[c] = fmaxf(mloc[c], __shfl_xor(mloc[c], o, 64));
+        if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6][c] = mloc[c];
+      }
+      __syncthreads();
+      if (threadIdx.x < NCC) {
+        float m = s_red[0][threadIdx.x];
+#pragma unroll
+        for (int wv = 1; wv < NI_BLOCK / 64; ++wv) m = fmaxf(m, s_red[wv][threadIdx.x]);
+        int e = 0;
+        if (m > 0.f && n_in > 0) {
+          int q, kb = 0;
+          frexpf(m, &q);                       // m < 2^q
+          while ((1 << kb) < n_in) ++kb;       // n_in <= 2^kb
+          e = 30 - q - kb;
+          e = e > 126 ? 126 : (e < -126 ? -126 : e);
+        }
+        s_scale[threadIdx.x] = ldexpf(1.f, e);
+        s_inv[threadIdx.x] = ldexpf(1.f, -e);
+      }
     }
     __syncthreads();
 
@@ -260,6 +374,37 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
 #pragma unroll
         for (int c = 0; c < NCC; ++c) acc[c] = 0.f;
         if (inb[k]) {
+          // z innermost: consecutive taps are adjacent LDS words (paired
+          // reads); the sum order differs from pp_sample's (z outer) only
+          // by rounding
+          const float *bp = box + off[k];
+#pragma unroll
+          for (int kx = 0; kx < NT; ++kx)
+#pragma unroll
+            for (int ky = 0; ky < NT; ++ky) {
+              const float wxy = w[k][0][kx] * w[k][1][ky];
+#pragma unroll
+              for (int kz = 0; kz < NT; ++kz) {
+                const float ww = wxy * w[k][2][kz];
+                const int o = kx * DYZ + ky * DZ + kz;
+#pragma unroll
+                for (int c = 0; c < NCC; ++c) acc[c] += bp[c * VOL + o] * ww;
+              }
+              // bound the LDS reads in flight to one row of taps
+              __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        float *op = a.out + (int)(n * a.out_s[0] + (int64_t)c0 * oC + px[k] * a.out_s[2] + py[k] * a.out_s[3] +
+                                  pz[k] * a.out_s[4]);
+#pragma unroll
+        for (int c = 0; c < NCC; ++c)
+          if (c < nc) op[c * oC] = acc[c];
+      } else if (MODE == TM_SGRAD) {
+        if (!valid[k]) continue;
+        float ax[NCC], ay[NCC], az[NCC];
+#pragma unroll
+        for (int c = 0; c < NCC; ++c) ax[c] = ay[c] = az[c] = 0.f;
+        if (inb[k]) {
           const float *bp = box + off[k];
 #pragma unroll
           for (int kz = 0; kz < NT; ++kz)
@@ -267,11 +412,17 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
             for (int ky = 0; ky < NT; ++ky)
 #pragma unroll
               for (int kx = 0; kx < NT; ++kx) {
-                const float ww = (w[k][0][kx] * w[k][1][ky]) * w[k][2][kz];
+                const float wx = w[k][0][kx], wy = w[k][1][ky], wz = w[k][2][kz];
+                const float gx = (gw[k][0][kx] * wy) * wz, gy = (wx * gw[k][1][ky]) * wz,
+                            gz = (wx * wy) * gw[k][2][kz];
                 const int o = kx * DYZ + ky * DZ + kz;
 #pragma unroll
-                for (int c = 0; c < NCC; ++c) acc[c] += bp[c * VOL + o] * ww;
-                // bound the LDS reads in flight to one row of taps
+                for (int c = 0; c < NCC; ++c) {
+                  const float v = bp[c * VOL + o];
+                  ax[c] += v * gx;
+                  ay[c] += v * gy;
+                  az[c] += v * gz;
+                }
                 if (kx == NT - 1) __builtin_amdgcn_sched_barrier(0);
               }
         }
@@ -279,18 +430,49 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
                                   pz[k] * a.out_s[4]);
 #pragma unroll
         for (int c = 0; c < NCC; ++c)
-          if (c < nc) op[c * oC] = acc[c];
-      } else {
+          if (c < nc) {
+            op[c * oC] = ax[c];
+            op[c * oC + oK] = ay[c];
+            op[c * oC + 2 * oK] = az[c];
+          }
+      } else if (MODE == TM_GGRAD || MODE == TM_GGRAD1) {
         if (!inb[k]) continue;
-        float tv[NCC];
-        if (MODE == TM_PUSH) {
+        float tvv[NCC];
+        if (MODE == TM_GGRAD) {
           const float *tp = a.trgt + (int)(n * a.trgt_s[0] + (int64_t)c0 * a.trgt_s[1] + px[k] * a.trgt_s[2] +
                                            py[k] * a.trgt_s[3] + pz[k] * a.trgt_s[4]);
           const int tC = (int)a.trgt_s[1];
 #pragma unroll
-          for (int c = 0; c < NCC; ++c) tv[c] = c < nc ? tp[c * tC] : 0.f;
+          for (int c = 0; c < NCC; ++c) tvv[c] = c < nc ? tp[c * tC] : 0.f;
+        } else {
+#pragma unroll
+          for (int c = 0; c < NCC; ++c) tvv[c] = c < nc ? 1.f : 0.f;
         }
-        float *bp = box + off[k];
+        const float *bp = box + off[k];
+#pragma unroll
+        for (int kz = 0; kz < NT; ++kz)
+#pragma unroll
+          for (int ky = 0; ky < NT; ++ky)
+#pragma unroll
+            for (int kx = 0; kx < NT; ++kx) {
+              const float wx = w[k][0][kx], wy = w[k][1][ky], wz = w[k][2][kz];
+              const float gx = (gw[k][0][kx] * wy) * wz, gy = (wx * gw[k][1][ky]) * wz,
+                          gz = (wx * wy) * gw[k][2][kz];
+              const int o = kx * DYZ + ky * DZ + kz;
+              float dot = 0.f;
+#pragma unroll
+              for (int c = 0; c < NCC; ++c) dot += bp[c * VOL + o] * tvv[c];
+              og[k][0] += gx * dot;
+              og[k][1] += gy * dot;
+              og[k][2] += gz * dot;
+              if (kx == NT - 1) __builtin_amdgcn_sched_barrier(0);
+            }
+      } else {
+        if (!inb[k]) continue;
+        float sv[NCC];
+#pragma unroll
+        for (int c = 0; c < NCC; ++c) sv[c] = tv[k][c] * s_scale[c];   // exact (power of two)
+        int *bp = (int *)box + off[k];
 #pragma unroll
         for (int kz = 0; kz < NT; ++kz)
 #pragma unroll
@@ -299,33 +481,51 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
             for (int kx = 0; kx < NT; ++kx) {
               const float ww = (w[k][0][kx] * w[k][1][ky]) * w[k][2][kz];
               const int o = kx * DYZ + ky * DZ + kz;
-              if (MODE == TM_COUNT) {
-                atomicAdd(bp + o, ww);
-              } else {
 #pragma unroll
-                for (int c = 0; c < NCC; ++c) atomicAdd(bp + c * VOL + o, ww * tv[c]);
-              }
+              for (int c = 0; c < NCC; ++c) atomicAdd(bp + c * VOL + o, (int)rintf(ww * sv[c]));
             }
       }
     }
 
-    if (MODE != TM_PULL) {
+    if (!GATHER) {
       // ---- add the box to the output, folded and signed ----
       __syncthreads();
       float *out = a.out + (int)(n * a.out_s[0] + (int64_t)c0 * oC);
-      for (int i = threadIdx.x; i < nbox; i += NI_BLOCK) {
-        const float v = box[i];
-        if (v == 0.f) continue;
-        const int c = i / VOL;
-        const int rr = i - c * VOL;
-        const int bx = rr / DYZ;
-        const int rem = rr - bx * DYZ;
-        const int by = rem / DZ;
-        const int bz = rem - by * DZ;
+      const int *ibox = (const int *)box;
+      const int E2 = ext[2], E12 = ext[1] * ext[2], E = ext[0] * E12;
+      const float rE = 1.f / (float)E, r12 = 1.f / (float)E12, r2 = 1.f / (float)E2;
+      const int nflush = nc * E;
+      for (int i = threadIdx.x; i < nflush; i += NI_BLOCK) {
+        int c = (int)((float)i * rE);
+        c -= c * E > i;
+        c += (c + 1) * E <= i;
+        const int rr = i - c * E;
+        int bx = (int)((float)rr * r12);
+        bx -= bx * E12 > rr;
+        bx += (bx + 1) * E12 <= rr;
+        const int rem = rr - bx * E12;
+        int by = (int)((float)rem * r2);
+        by -= by * E2 > rem;
+        by += (by + 1) * E2 <= rem;
+        const int bz = rem - by * E2;
+        const int iv = ibox[c * VOL + (bx * DY + by) * DZ + bz];
+        if (iv == 0) continue;
+        const float v = (float)iv * s_inv[c];
         const float sg = fsg[bx] * fsg[B::DX + by] * fsg[B::DX + DY + bz];
         if (sg != 0.f) atomic_add(out + c * oC + fofs[bx] + fofs[B::DX + by] + fofs[B::DX + DY + bz], v * sg);
       }
     }
+  }
+  if (MODE == TM_GGRAD || MODE == TM_GGRAD1) {
+#pragma unroll
+    for (int k = 0; k < TL_PER_THREAD; ++k)
+      if (valid[k]) {
+        float *gp = a.gout + (int)(n * a.gout_s[0] + px[k] * a.gout_s[1] + py[k] * a.gout_s[2] + pz[k] * a.gout_s[3]);
+        const int gD = (int)a.gout_s[4];
+        gp[0] = og[k][0];
+        gp[gD] = og[k][1];
+        gp[2 * gD] = og[k][2];
+      }
   }
 }
 
@@ -369,8 +569,8 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_fallback(PPArgs<float> a, int n
       if (ok) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          float wd[NT];
-          const int l = tl_taps<ORD>(xs[d], wd);
+          float wd[NT], gd[NT];
+          const int l = tl_taps<ORD>(xs[d], wd, gd);
           mn[d] = min(mn[d], l);
           mx[d] = max(mx[d], l);
         }
@@ -417,12 +617,16 @@ static void launch_tile(const PPArgs<float> &a, hipStream_t st) {
   const int64_t nb = a.N * (int64_t)ntx * nty * ntz;
   const dim3 g((unsigned)nb), b(NI_BLOCK);
   constexpr int M = TileBox<ORD>::MAX_NCC;
-  if (MODE == TM_COUNT || a.C == 1)
+  if constexpr (MODE == TM_COUNT) {
     hipLaunchKernelGGL((tile_kernel<ORD, MODE, 1>), g, b, 0, st, a, ntx, nty, ntz, nb);
-  else if (a.C == 2 || M == 2)
-    hipLaunchKernelGGL((tile_kernel<ORD, MODE, (M < 2 ? M : 2)>), g, b, 0, st, a, ntx, nty, ntz, nb);
-  else
-    hipLaunchKernelGGL((tile_kernel<ORD, MODE, M>), g, b, 0, st, a, ntx, nty, ntz, nb);
+  } else {
+    if (a.C == 1)
+      hipLaunchKernelGGL((tile_kernel<ORD, MODE, 1>), g, b, 0, st, a, ntx, nty, ntz, nb);
+    else if (a.C == 2 || M == 2)
+      hipLaunchKernelGGL((tile_kernel<ORD, MODE, (M < 2 ? M : 2)>), g, b, 0, st, a, ntx, nty, ntz, nb);
+    else
+      hipLaunchKernelGGL((tile_kernel<ORD, MODE, M>), g, b, 0, st, a, ntx, nty, ntz, nb);
+  }
   hipLaunchKernelGGL((tile_fallback<ORD, MODE>), g, b, 0, st, a, ntx, nty, ntz, nb);
 }
 

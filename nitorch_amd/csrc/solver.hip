@@ -1090,7 +1090,7 @@ int ni_fmg_solve(int32_t dtype, int64_t N, const int64_t *shape, const void *hes
   double rel = 0;
   auto loop = [&](auto &F) -> int {
     using T = typename std::remove_reference<decltype(*F.x[0])>::type;
-    fmg_run(F, nb_cycles);
+    if (nb_cycles > 0) fmg_run(F, nb_cycles);   // else: V-cycles from the given solution
     while (true) {
       launch_fill<double>(norms, 2, 0.0, st);
       launch_reg<T>(F.x[0], F.h[0], F.CC, F.g0, F.r[0], F.G[0], F.S[0], F.N, 1, norms, st);

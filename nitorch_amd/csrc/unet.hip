@@ -1,0 +1,522 @@
+// BF16 kernels of the learned preconditioner's 3-D U-Net (no counterpart in
+// the reference): 3x3x3 convolutions with FP32 accumulation on the gfx950
+// matrix cores, their backward passes, and the pooling / upsampling / concat
+// / packing kernels around them.
+//
+// Layout: activations channels-last [N, D, H, W, C] in bf16, C a multiple of
+// 16 (the 9 input channels are padded to 16 by the packing kernel).
+// Weights of a layer CI -> CO: [CO][KP] bf16 with K = tap * CI + ci,
+// tap = 9 dx + 3 dy + dz (dx, dy, dz in 0..2 <-> offsets -1..1), KP = 27 CI
+// rounded up to 32 and zero-padded.  Zero padding of 1 voxel, stride 1.
+//
+// Convolution as an implicit GEMM on v_mfma_f32_16x16x32_bf16:
+//   rows (M) = 16 consecutive z positions of one (n, x, y) row,
+//   cols (N) = 16 output channels, depth (K) = 32 (tap, ci) pairs.
+// A lane supplies row (lane % 16) and the 8 consecutive K elements
+// 8 * (lane / 16) .. +7 of its operand, i.e. 8 consecutive channels of one
+// tap of one voxel (one 16-byte load); accumulator element r of a lane is
+// row 4 * (lane / 16) + r, column lane % 16.  A wave computes 4 row blocks
+// (64 positions) x all output-channel blocks.
+//
+// Backward data is the same kernel run on the activation gradient with the
+// flipped, transposed weights (prepared on the host side); backward weights
+// reduce over positions with the same instruction (rows = output channels,
+// cols = (tap, ci), depth = 32 positions), FP32 partial sums added with
+// atomics.
+#include <cstdint>
+
+#include "common.h"
+#include "nitorch_amd.h"
+
+namespace ni_amd {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+#define UN_BLOCK 256
+#define UN_MB 4          // 16-position row blocks per wave
+#define UN_SLOPE 0.1f    // leaky ReLU negative slope
+
+template <int CI>
+struct KPad {
+  static constexpr int value = (27 * CI + 31) / 32 * 32;
+};
+
+__device__ __forceinline__ bf16x8 zero8() {
+  bf16x8 z;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) z[j] = (bf16)0.f;
+  return z;
+}
+
+// ------------------------------------------------------------------------
+// y = act(conv(x, w) + b).  ACT: 0 none, 1 leaky ReLU.  COS output channels
+// are stored (COS <= CO; a 3-channel head is computed with CO = 16 and zero
+// weights in the padding), as bf16 or fp32.
+// ------------------------------------------------------------------------
+template <int CI, int CO, int COS, int ACT, bool OUTF32>
+__global__ __launch_bounds__(UN_BLOCK) void conv3_fwd(const bf16 *__restrict__ X, const bf16 *__restrict__ Wk,
+                                                      const float *__restrict__ bias, void *__restrict__ Yv,
+                                                      int N, int D, int H, int W) {
+  constexpr int KP = KPad<CI>::value;
+  constexpr int CB = CO / 16;
+  static_assert(CI % 16 == 0 && CO % 16 == 0 && COS <= CO, "channel counts");
+  const int WB = (W + 15) / 16;
+  const int64_t total = (int64_t)N * D * H * WB;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = lane & 15, ks = lane >> 4;
+  const int64_t pb0 = ((int64_t)blockIdx.x * (UN_BLOCK / 64) + wave) * UN_MB;
+
+  int px[UN_MB], py[UN_MB], pz[UN_MB], pn[UN_MB];
+  bool pv[UN_MB];
+#pragma unroll
+  for (int m = 0; m < UN_MB; ++m) {
+    const int64_t pb = pb0 + m;
+    pv[m] = pb < total;
+    int64_t t = pv[m] ? pb : 0;
+    const int zb = (int)(t % WB); t /= WB;
+    py[m] = (int)(t % H); t /= H;
+    px[m] = (int)(t % D);
+    pn[m] = (int)(t / D);
+    pz[m] = zb * 16 + row;
+  }
+
+  f32x4 acc[UN_MB][CB];
+#pragma unroll
+  for (int m = 0; m < UN_MB; ++m)
+#pragma unroll
+    for (int c = 0; c < CB; ++c) acc[m][c] = (f32x4){0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll 2
+  for (int kc = 0; kc < KP / 32; ++kc) {
+    const int k0 = kc * 32 + ks * 8;
+    const int tap = k0 / CI, ci = k0 - tap * CI;
+    const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+    bf16x8 b[CB];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) b[c] = *(const bf16x8 *)(Wk + (size_t)(c * 16 + row) * KP + k0);
+#pragma unroll
+    for (int m = 0; m < UN_MB; ++m) {
+      const int xx = px[m] + dx, yy = py[m] + dy, zz = pz[m] + dz;
+      bf16x8 a = zero8();
+      if (pv[m] && tap < 27 && xx >= 0 && xx < D && yy >= 0 && yy < H && zz >= 0 && zz < W)
+        a = *(const bf16x8 *)(X + ((((size_t)pn[m] * D + xx) * H + yy) * W + zz) * CI + ci);
+#pragma unroll
+      for (int c = 0; c < CB; ++c) acc[m][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[c], acc[m][c], 0, 0, 0);
+    }
+  }
+
+#pragma unroll
+  for (int m = 0; m < UN_MB; ++m) {
+    if (!pv[m]) continue;
+    const int zb0 = pz[m] - row;
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      const int co = c * 16 + row;
+      if (co >= COS) continue;
+      const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int z = zb0 + ks * 4 + r;
+        if (z >= W) continue;
+        float v = acc[m][c][r] + bb;
+        if (ACT == 1) v = v > 0.f ? v : UN_SLOPE * v;
+        const size_t o = ((((size_t)pn[m] * D + px[m]) * H + py[m]) * W + z) * COS + co;
+        if (OUTF32) ((float *)Yv)[o] = v;
+        else ((bf16 *)Yv)[o] = (bf16)v;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// dW[co][tap * CI + ci] += sum_p dY[p][co] * X[p + off(tap)][ci]  (fp32, [CO][KP])
+// One wave: one 16-channel block of CO, UN_NT column blocks of (tap, ci),
+// `rows` consecutive (n, x, y) rows, all z.
+// ------------------------------------------------------------------------
+#define UN_NT 4
+
+template <int CI, int CO>
+__global__ __launch_bounds__(UN_BLOCK) void conv3_bwdw(const bf16 *__restrict__ X, const bf16 *__restrict__ dY,
+                                                       float *__restrict__ dW, int N, int D, int H, int W,
+                                                       int rows) {
+  constexpr int KP = KPad<CI>::value;
+  constexpr int NCB = 27 * CI / 16;
+  constexpr int NG = (NCB + UN_NT - 1) / UN_NT;
+  constexpr int COB = CO / 16;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l16 = lane & 15, ks = lane >> 4;
+  const int64_t nrows = (int64_t)N * D * H;
+  const int64_t nranges = (nrows + rows - 1) / rows;
+  int64_t task = (int64_t)blockIdx.x * (UN_BLOCK / 64) + wave;
+  if (task >= nranges * COB * NG) return;
+  const int grp = (int)(task % NG); task /= NG;
+  const int cob = (int)(task % COB);
+  const int64_t rr = task / COB;
+  const int co = cob * 16 + l16;
+
+  int tap[UN_NT], ci[UN_NT], ox[UN_NT], oy[UN_NT], oz[UN_NT];
+  bool cv[UN_NT];
+#pragma unroll
+  for (int t = 0; t < UN_NT; ++t) {
+    const int col = (grp * UN_NT + t) * 16 + l16;
+    cv[t] = col < 27 * CI;
+    tap[t] = cv[t] ? col / CI : 0;
+    ci[t] = cv[t] ? col - tap[t] * CI : 0;
+    ox[t] = tap[t] / 9 - 1; oy[t] = (tap[t] / 3) % 3 - 1; oz[t] = tap[t] % 3 - 1;
+  }
+  f32x4 acc[UN_NT];
+#pragma unroll
+  for (int t = 0; t < UN_NT; ++t) acc[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+
+  const int64_t r_end = (rr + 1) * rows < nrows ? (rr + 1) * rows : nrows;
+  for (int64_t r = rr * rows; r < r_end; ++r) {
+    const int y = (int)(r % H), x = (int)((r / H) % D), n = (int)(r / ((int64_t)H * D));
+    for (int z0 = 0; z0 < W; z0 += 32) {
+      bf16x8 a;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int z = z0 + ks * 8 + j;
+        a[j] = z < W ? dY[((((size_t)n * D + x) * H + y) * W + z) * CO + co] : (bf16)0.f;
+      }
+#pragma unroll
+      for (int t = 0; t < UN_NT; ++t) {
+        bf16x8 b;
+        const int xx = x + ox[t], yy = y + oy[t];
+        const bool rowok = cv[t] && xx >= 0 && xx < D && yy >= 0 && yy < H;
+        const size_t base = (((size_t)n * D + (rowok ? xx : 0)) * H + (rowok ? yy : 0)) * W;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int zz = z0 + ks * 8 + j + oz[t];
+          const bool ok = rowok && (z0 + ks * 8 + j) < W && zz >= 0 && zz < W;
+          b[j] = ok ? X[(base + zz) * CI + ci[t]] : (bf16)0.f;
+        }
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < UN_NT; ++t) {
+    const int col = (grp * UN_NT + t) * 16 + l16;
+    if (col >= 27 * CI) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int cr = cob * 16 + ks * 4 + q;
+      atomicAdd(dW + (size_t)cr * KP + col, acc[t][q]);
+    }
+  }
+}
+
+// db[co] += sum_p dY[p][co]
+template <int CO>
+__global__ __launch_bounds__(UN_BLOCK) void bias_grad(const bf16 *__restrict__ dY, float *__restrict__ db, int64_t P,
+                                                      int64_t per_block) {
+  __shared__ float part[UN_BLOCK];
+  const int co = threadIdx.x % CO, lanes = UN_BLOCK / CO;
+  const int64_t p0 = (int64_t)blockIdx.x * per_block;
+  const int64_t p1 = p0 + per_block < P ? p0 + per_block : P;
+  float s = 0.f;
+  for (int64_t p = p0 + threadIdx.x / CO; p < p1; p += lanes) s += (float)dY[p * CO + co];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < CO) {
+    float t = 0.f;
+    for (int k = threadIdx.x; k < lanes * CO; k += CO) t += part[k];
+    atomicAdd(db + threadIdx.x, t);
+  }
+}
+
+// ------------------------------------------------------------------------
+// elementwise kernels (bf16, channels-last)
+// ------------------------------------------------------------------------
+// dpre = dy * (y > 0 ? 1 : slope)
+__global__ void lrelu_bwd(const bf16 *__restrict__ dy, const bf16 *__restrict__ y, bf16 *__restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * UN_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const float g = (float)dy[i];
+  out[i] = (bf16)((float)y[i] > 0.f ? g : UN_SLOPE * g);
+}
+
+// 2x2x2 average pool: x [N, D, H, W, C] -> y [N, D/2, H/2, W/2, C]
+__global__ void avgpool2_fwd(const bf16 *__restrict__ x, bf16 *__restrict__ y, int N, int D, int H, int W, int C) {
+  const int64_t i = (int64_t)blockIdx.x * UN_BLOCK + threadIdx.x;
+  const int d2 = D / 2, h2 = H / 2, w2 = W / 2;
+  const int64_t n_out = (int64_t)N * d2 * h2 * w2 * C;
+  if (i >= n_out) return;
+  int64_t t = i;
+  const int c = (int)(t % C); t /= C;
+  const int z = (int)(t % w2); t /= w2;
+  const int yy = (int)(t % h2); t /= h2;
+  const int xx = (int)(t % d2);
+  const int n = (int)(t / d2);
+  float s = 0.f;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 2; ++e)
+        s += (float)x[((((int64_t)n * D + 2 * xx + a) * H + 2 * yy + b) * W + 2 * z + e) * C + c];
+  y[i] = (bf16)(s * 0.125f);
+}
+
+// dx[child] = dy[parent] / 8
+__global__ void avgpool2_bwd(const bf16 *__restrict__ dy, bf16 *__restrict__ dx, int N, int D, int H, int W, int C) {
+  const int64_t i = (int64_t)blockIdx.x * UN_BLOCK + threadIdx.x;
+  const int64_t n_in = (int64_t)N * D * H * W * C;
+  if (i >= n_in) return;
+  int64_t t = i;
+  const int c = (int)(t % C); t /= C;
+  const int z = (int)(t % W); t /= W;
+  const int yy = (int)(t % H); t /= H;
+  const int xx = (int)(t % D);
+  const int n = (int)(t / D);
+  dx[i] = (bf16)((float)dy[((((int64_t)n * (D / 2) + xx / 2) * (H / 2) + yy / 2) * (W / 2) + z / 2) * C + c] * 0.125f);
+}
+
+// out [N, D, H, W, C1 + C2] = concat(nearest_up2(a [N, D/2, H/2, W/2, C1]), skip [N, D, H, W, C2])
+__global__ void up2_concat_fwd(const bf16 *__restrict__ a, const bf16 *__restrict__ skip, bf16 *__restrict__ out,
+                               int N, int D, int H, int W, int C1, int C2) {
+  const int64_t i = (int64_t)blockIdx.x * UN_BLOCK + threadIdx.x;
+  const int C = C1 + C2;
+  const int64_t n_out = (int64_t)N * D * H * W * C;
+  if (i >= n_out) return;
+  int64_t t = i;
+  const int c = (int)(t % C); t /= C;
+  const int64_t p = t;
+  const int z = (int)(t % W); t /= W;
+  const int yy = (int)(t % H); t /= H;
+  const int xx = (int)(t % D);
+  const int n = (int)(t / D);
+  if (c < C1)
+    out[i] = a[((((int64_t)n * (D / 2) + xx / 2) * (H / 2) + yy / 2) * (W / 2) + z / 2) * C1 + c];
+  else
+    out[i] = skip[p * C2 + (c - C1)];
+}
+
+// backward of up2_concat_fwd: da = sum over the 8 children, dskip = slice
+__global__ void up2_concat_bwd(const bf16 *__restrict__ dout, bf16 *__restrict__ da, bf16 *__restrict__ dskip,
+                               int N, int D, int H, int W, int C1, int C2) {
+  const int64_t i = (int64_t)blockIdx.x * UN_BLOCK + threadIdx.x;
+  const int C = C1 + C2;
+  const int d2 = D / 2, h2 = H / 2, w2 = W / 2;
+  const int64_t n_a = (int64_t)N * d2 * h2 * w2 * C1;
+  const int64_t n_s = (int64_t)N * D * H * W * C2;
+  if (i < n_a) {
+    int64_t t = i;
+    const int c = (int)(t % C1); t /= C1;
+    const int z = (int)(t % w2); t /= w2;
+    const int yy = (int)(t % h2); t /= h2;
+    const int xx = (int)(t % d2);
+    const int n = (int)(t / d2);
+    float s = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      s += (float)dout[((((int64_t)n * D + 2 * xx + (q >> 2)) * H + 2 * yy + ((q >> 1) & 1)) * W + 2 * z + (q & 1)) * C + c];
+    da[i] = (bf16)s;
+  } else if (i < n_a + n_s) {
+    const int64_t j = i - n_a;
+    const int c = (int)(j % C2);
+    dskip[j] = dout[(j / C2) * C + C1 + c];
+  }
+}
+
+// network input: [N, V, 16] bf16 = (g * sg (3 ch), h * sh (CC ch), 0 ...) from
+// channel-first fp32 g [N, 3, V], h [N, CC, V]
+__global__ void pack_input(const float *__restrict__ g, const float *__restrict__ h, int CC, bf16 *__restrict__ out,
+                           int N, int64_t V, float sg, float sh) {
+  const int64_t i = (int64_t)blockIdx.x * UN_BLOCK + threadIdx.x;
+  if (i >= (int64_t)N * V) return;
+  const int64_t n = i / V, v = i - n * V;
+  bf16 o[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) o[c] = (bf16)0.f;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) o[c] = (bf16)(g[(n * 3 + c) * V + v] * sg);
+  for (int c = 0; c < CC && c < 6; ++c) o[3 + c] = (bf16)(h[(n * CC + c) * V + v] * sh);
+  bf16x8 *dst = (bf16x8 *)(out + i * 16);
+  bf16x8 lo, hi;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) { lo[c] = o[c]; hi[c] = o[8 + c]; }
+  dst[0] = lo;
+  dst[1] = hi;
+}
+
+// x [N, 3, V] (fp32, channel-first) = scale * y [N, V, 3] (+ x if accumulate)
+__global__ void unpack_output(const float *__restrict__ y, float *__restrict__ x, int N, int64_t V, float scale,
+                              int accumulate) {
+  const int64_t i = (int64_t)blockIdx.x * UN_BLOCK + threadIdx.x;
+  if (i >= (int64_t)N * 3 * V) return;
+  const int64_t n = i / (3 * V), r = i - n * 3 * V, c = r / V, v = r - c * V;
+  const float val = scale * y[(n * V + v) * 3 + c];
+  x[i] = accumulate ? x[i] + val : val;
+}
+
+static inline unsigned nblk(int64_t n) { return (unsigned)((n + UN_BLOCK - 1) / UN_BLOCK); }
+
+// ---------------------------- dispatch ----------------------------------
+#define UN_FWD_CASES(X_)                                                      \
+  X_(16, 16) X_(16, 32) X_(32, 32) X_(32, 64) X_(64, 64) X_(96, 32) X_(48, 16) \
+  X_(32, 16) X_(64, 32) X_(32, 96) X_(16, 48)
+
+template <int CI, int CO>
+static void launch_fwd(const bf16 *x, const bf16 *w, const float *b, void *y, int N, int D, int H, int W, int act,
+                       int out_f32, int cos, hipStream_t st) {
+  const int64_t pbs = (int64_t)N * D * H * ((W + 15) / 16);
+  const int64_t waves = (pbs + UN_MB - 1) / UN_MB;
+  const dim3 g((unsigned)((waves + 3) / 4)), bl(UN_BLOCK);
+  if (out_f32) {
+    if constexpr (CO == 16)
+      if (cos == 3) {
+        hipLaunchKernelGGL((conv3_fwd<CI, 16, 3, 0, true>), g, bl, 0, st, x, w, b, y, N, D, H, W);
+        return;
+      }
+    hipLaunchKernelGGL((conv3_fwd<CI, CO, CO, 0, true>), g, bl, 0, st, x, w, b, y, N, D, H, W);
+  } else if (act) {
+    hipLaunchKernelGGL((conv3_fwd<CI, CO, CO, 1, false>), g, bl, 0, st, x, w, b, y, N, D, H, W);
+  } else {
+    hipLaunchKernelGGL((conv3_fwd<CI, CO, CO, 0, false>), g, bl, 0, st, x, w, b, y, N, D, H, W);
+  }
+}
+
+template <int CI, int CO>
+static void launch_bwdw(const bf16 *x, const bf16 *dy, float *dw, int N, int D, int H, int W, hipStream_t st) {
+  constexpr int NG = (27 * CI / 16 + UN_NT - 1) / UN_NT;
+  const int64_t nrows = (int64_t)N * D * H;
+  // rows per wave: enough work per wave to amortise the atomics, enough waves to fill the chip
+  int rows = (int)(nrows * CO / 16 * NG / 4096);
+  rows = rows < 1 ? 1 : (rows > 64 ? 64 : rows);
+  const int64_t tasks = (nrows + rows - 1) / rows * (CO / 16) * NG;
+  hipLaunchKernelGGL((conv3_bwdw<CI, CO>), dim3((unsigned)((tasks + 3) / 4)), dim3(UN_BLOCK), 0, st, x, dy, dw, N, D,
+                     H, W, rows);
+}
+
+}  // namespace ni_amd
+
+using namespace ni_amd;
+
+extern "C" {
+
+int ni_unet_conv3_fwd(int64_t N, const int64_t *shape, int32_t CI, int32_t CO, const void *x, const void *w,
+                      const float *bias, void *y, int32_t act, int32_t out_f32, int32_t out_channels, void *stream) {
+  set_error("");
+  NI_CHECK(x && w && y && shape, "unet_conv3_fwd: missing argument");
+  NI_CHECK(N >= 1 && shape[0] >= 1 && shape[1] >= 1 && shape[2] >= 1, "unet_conv3_fwd: bad shape");
+  NI_CHECK(out_channels == CO || (out_f32 && CO == 16 && out_channels == 3),
+           "unet_conv3_fwd: out_channels must equal CO (or 3 with CO = 16, fp32 output)");
+  NI_CHECK(!(out_f32 && act), "unet_conv3_fwd: fp32 output is for the linear head");
+  const int D = (int)shape[0], H = (int)shape[1], W = (int)shape[2];
+  hipStream_t st = (hipStream_t)stream;
+  bool done = false;
+#define UN_X(ci, co)                                                                                   \
+  if (!done && CI == ci && CO == co) {                                                                 \
+    launch_fwd<ci, co>((const bf16 *)x, (const bf16 *)w, bias, y, (int)N, D, H, W, act, out_f32,       \
+                       out_channels, st);                                                              \
+    done = true;                                                                                       \
+  }
+  UN_FWD_CASES(UN_X)
+#undef UN_X
+  NI_CHECK(done, "unet_conv3_fwd: unsupported channel pair %d -> %d", CI, CO);
+  NI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int ni_unet_conv3_bwd_weight(int64_t N, const int64_t *shape, int32_t CI, int32_t CO, const void *x, const void *dy,
+                             float *dw, float *db, void *stream) {
+  set_error("");
+  NI_CHECK(x && dy && dw && shape, "unet_conv3_bwd_weight: missing argument");
+  const int D = (int)shape[0], H = (int)shape[1], W = (int)shape[2];
+  hipStream_t st = (hipStream_t)stream;
+  bool done = false;
+#define UN_X(ci, co)                                                                                   \
+  if (!done && CI == ci && CO == co) {                                                                 \
+    launch_bwdw<ci, co>((const bf16 *)x, (const bf16 *)dy, dw, (int)N, D, H, W, st);                   \
+    done = true;                                                                                       \
+  }
+  UN_FWD_CASES(UN_X)
+#undef UN_X
+  NI_CHECK(done, "unet_conv3_bwd_weight: unsupported channel pair %d -> %d", CI, CO);
+  if (db) {
+    const int64_t P = N * shape[0] * shape[1] * shape[2];
+    const int64_t per = 4096;
+    const unsigned nb = (unsigned)((P + per - 1) / per);
+    switch (CO) {
+      case 16: hipLaunchKernelGGL(bias_grad<16>, dim3(nb), dim3(UN_BLOCK), 0, st, (const bf16 *)dy, db, P, per); break;
+      case 32: hipLaunchKernelGGL(bias_grad<32>, dim3(nb), dim3(UN_BLOCK), 0, st, (const bf16 *)dy, db, P, per); break;
+      case 48: hipLaunchKernelGGL(bias_grad<48>, dim3(nb), dim3(UN_BLOCK), 0, st, (const bf16 *)dy, db, P, per); break;
+      case 64: hipLaunchKernelGGL(bias_grad<64>, dim3(nb), dim3(UN_BLOCK), 0, st, (const bf16 *)dy, db, P, per); break;
+      case 96: hipLaunchKernelGGL(bias_grad<96>, dim3(nb), dim3(UN_BLOCK), 0, st, (const bf16 *)dy, db, P, per); break;
+      default: NI_CHECK(false, "unet_conv3_bwd_weight: unsupported CO %d for the bias gradient", CO);
+    }
+  }
+  NI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int ni_unet_lrelu_bwd(int64_t n, const void *dy, const void *y, void *out, void *stream) {
+  set_error("");
+  NI_CHECK(dy && y && out, "unet_lrelu_bwd: missing argument");
+  if (n > 0)
+    hipLaunchKernelGGL(lrelu_bwd, dim3(nblk(n)), dim3(UN_BLOCK), 0, (hipStream_t)stream, (const bf16 *)dy,
+                       (const bf16 *)y, (bf16 *)out, n);
+  NI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int ni_unet_avgpool2(int32_t backward, int64_t N, const int64_t *shape, int32_t C, const void *in, void *out,
+                     void *stream) {
+  set_error("");
+  NI_CHECK(in && out && shape, "unet_avgpool2: missing argument");
+  const int D = (int)shape[0], H = (int)shape[1], W = (int)shape[2];
+  NI_CHECK(D % 2 == 0 && H % 2 == 0 && W % 2 == 0, "unet_avgpool2: even sizes required");
+  hipStream_t st = (hipStream_t)stream;
+  if (!backward)
+    hipLaunchKernelGGL(avgpool2_fwd, dim3(nblk(N * D * H * W * C / 8)), dim3(UN_BLOCK), 0, st, (const bf16 *)in,
+                       (bf16 *)out, (int)N, D, H, W, C);
+  else
+    hipLaunchKernelGGL(avgpool2_bwd, dim3(nblk(N * D * H * W * C)), dim3(UN_BLOCK), 0, st, (const bf16 *)in,
+                       (bf16 *)out, (int)N, D, H, W, C);
+  NI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int ni_unet_up2_concat(int32_t backward, int64_t N, const int64_t *shape, int32_t C1, int32_t C2, const void *a,
+                       const void *skip, void *out, void *stream) {
+  set_error("");
+  NI_CHECK(a && skip && out && shape, "unet_up2_concat: missing argument");
+  const int D = (int)shape[0], H = (int)shape[1], W = (int)shape[2];
+  NI_CHECK(D % 2 == 0 && H % 2 == 0 && W % 2 == 0, "unet_up2_concat: even sizes required");
+  hipStream_t st = (hipStream_t)stream;
+  if (!backward)
+    hipLaunchKernelGGL(up2_concat_fwd, dim3(nblk(N * D * H * W * (C1 + C2))), dim3(UN_BLOCK), 0, st, (const bf16 *)a,
+                       (const bf16 *)skip, (bf16 *)out, (int)N, D, H, W, C1, C2);
+  else  // a <- d(low-res input), skip <- d(skip), out = incoming gradient
+    hipLaunchKernelGGL(up2_concat_bwd, dim3(nblk(N * D * H * W * C1 / 8 + N * D * H * W * C2)), dim3(UN_BLOCK), 0,
+                       st, (const bf16 *)out, (bf16 *)a, (bf16 *)skip, (int)N, D, H, W, C1, C2);
+  NI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int ni_unet_pack_input(int64_t N, int64_t V, const float *g, const float *h, int32_t CC, void *out, float sg,
+                       float sh, void *stream) {
+  set_error("");
+  NI_CHECK(g && out, "unet_pack_input: missing argument");
+  NI_CHECK(h == nullptr || CC == 1 || CC == 3 || CC == 6, "unet_pack_input: CC must be 1, 3 or 6");
+  hipLaunchKernelGGL(pack_input, dim3(nblk(N * V)), dim3(UN_BLOCK), 0, (hipStream_t)stream, g, h, h ? CC : 0,
+                     (bf16 *)out, (int)N, V, sg, sh);
+  NI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int ni_unet_unpack_output(int64_t N, int64_t V, const float *y, float *x, float scale, int32_t accumulate,
+                          void *stream) {
+  set_error("");
+  NI_CHECK(y && x, "unet_unpack_output: missing argument");
+  hipLaunchKernelGGL(unpack_output, dim3(nblk(N * 3 * V)), dim3(UN_BLOCK), 0, (hipStream_t)stream, y, x, (int)N, V,
+                     scale, accumulate);
+  NI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"
