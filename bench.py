@@ -9,7 +9,8 @@ cubic B-splines (order 3), bound dct2, no extrapolation, inputs resident in
 HBM.  `value` = spatial samples processed per second over the step
 (2 x 4 x 192^3 samples per step), in Gvox/s.
 
-Also reported: per-op times, the order x bound sweep of pull/push/count/grad
+Also reported: per-op times (and the autograd backward of an order-3 dft
+pull on 2x3x128x160x128, both gradients), the order x bound sweep of pull/push/count/grad
 (orders 0/1/3/5 x dct2/dft/replicate/zero, one timed launch each after a
 warm-up), the roofline of the dominant kernel (algorithmic bytes per launch /
 average launch time, HIP events on the launch stream), and the reference's
@@ -106,6 +107,19 @@ def main():
     dom = max(ops, key=lambda k: ops[k]["ms"])
     achieved = gb_launch / (ops[dom]["ms"] / 1e3)
 
+    # backward_128: autograd backward of grid_pull (order 3, dft), both gradients
+    from nitorch_amd.bench_data import config3_inputs
+    x3, g3, d3 = (t.to(dev) for t in config3_inputs())
+    x3.requires_grad_(True)
+    g3.requires_grad_(True)
+    S3 = g3[..., 0].numel()
+    bwd = lambda: G.grid_pull_backward(d3, x3, g3, ["dft"], [3], 0)
+    bwd_ms, _ = timed(bwd, max(args.steps, 5), 1, stream)
+    bps3 = 24 + 12 * x3.shape[1]     # grid + grad_grid 24, input + grad + grad_input 12 per channel
+    ops["pull_backward_o3_dft"] = {"ms": bwd_ms, "gvox_s": S3 / bwd_ms / 1e6,
+                                   "hbm_frac": bps3 * S3 / (bwd_ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+    del x3, g3, d3
+
     sweep = {}
     if not args.no_sweep:
         for order in (0, 1, 3, 5):
@@ -187,6 +201,22 @@ def gn_solve(dev):
     best = min(("relax", "cg"), key=lambda k: out[k]["ms_to_1e-4"])
     out["plain_fmg_ms"] = out[best]["ms_to_1e-4"]
     out["plain_fmg_smoother"] = best
+    # learned preconditioner (trained weights committed with the package)
+    from nitorch_amd import learned as LN
+    wpath = LN.default_weights_path()
+    if os.path.exists(wpath):
+        net = LN.UNet.load(wpath, dev)
+        kw = dict(tol=1e-4, max_iter=200)
+        LN.pcg_learned(net, h, g, pen, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, rel, it, fb = LN.pcg_learned(net, h, g, pen, **kw)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3
+        out["learned"] = {"ms_to_1e-4": round(ms, 3), "rel_residual": rel, "pcg_iterations": it,
+                          "fmg_fallback": fb, "vs_plain_fmg": round(ms / out["plain_fmg_ms"], 3),
+                          "method": "flexible PCG, U-Net (3 levels, 16/32/64, BF16 MFMA) warm start + "
+                                    "preconditioner on block-Jacobi, FMG fallback"}
     return out
 
 

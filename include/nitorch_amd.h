@@ -187,17 +187,18 @@ int ni_fmg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hess
 /* Time-to-tolerance solve (no reference counterpart: the benchmark's "GN step
  * to 1e-4 relative residual"): ni_fmg_grid's full multigrid (skipped when
  * nb_cycles <= 0: `solution` is then the starting iterate), then V-cycles on
- * the finest level until ||g - (H+L)x|| / ||g|| <= tol or max_vcycles.
- * `norms` is a device buffer of 2 doubles; the call synchronises `stream`
- * once per V-cycle to read the residual.  Reports the final relative
- * residual and the number of extra V-cycles. */
+ * the finest level until ||g - (H+L)x|| / ||g|| <= tol or max_vcycles.  The
+ * workspace is ni_fmg_grid_workspace's; the call synchronises `stream` once
+ * per V-cycle to read the residual.  Reports the final relative residual
+ * and the number of extra V-cycles. */
 int ni_fmg_solve(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
                  const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
                  const double *penalty, int32_t nb_cycles, int32_t nb_iter, int32_t max_levels,
-                 int32_t use_cg, double tol, int32_t max_vcycles, double *norms, void *workspace,
+                 int32_t use_cg, double tol, int32_t max_vcycles, void *workspace,
                  int64_t workspace_bytes, double *out_rel, int32_t *out_vcycles, void *stream);
 
-/* Bytes of device workspace ni_pcg_grid needs (r, z, p and 4 N doubles). */
+/* Bytes of device workspace ni_pcg_grid needs (r, z, p and the partial sums
+ * of its scalars, 4 x 256 doubles per batch element). */
 int64_t ni_pcg_grid_workspace(int32_t dtype, int64_t N, const int64_t *shape);
 
 /* Replaces ni::pcg_grid (nitorch/_C/src/pcg.h:20-32, pcg_common.cpp:38-85,

@@ -378,7 +378,9 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
           // reads); the sum order differs from pp_sample's (z outer) only
           // by rounding
           const float *bp = box + off[k];
-#pragma unroll
+          // kx rolled: keeps the LDS reads of one (kx, ky) row in flight
+          // without the unrolled nest's register demand
+#pragma unroll 1
           for (int kx = 0; kx < NT; ++kx)
 #pragma unroll
             for (int ky = 0; ky < NT; ++ky) {
@@ -406,7 +408,7 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
         for (int c = 0; c < NCC; ++c) ax[c] = ay[c] = az[c] = 0.f;
         if (inb[k]) {
           const float *bp = box + off[k];
-#pragma unroll
+#pragma unroll 1
           for (int kz = 0; kz < NT; ++kz)
 #pragma unroll
             for (int ky = 0; ky < NT; ++ky)
@@ -449,7 +451,7 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
           for (int c = 0; c < NCC; ++c) tvv[c] = c < nc ? 1.f : 0.f;
         }
         const float *bp = box + off[k];
-#pragma unroll
+#pragma unroll 1
         for (int kz = 0; kz < NT; ++kz)
 #pragma unroll
           for (int ky = 0; ky < NT; ++ky)
@@ -460,8 +462,9 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
                           gz = (wx * wy) * gw[k][2][kz];
               const int o = kx * DYZ + ky * DZ + kz;
               float dot = 0.f;
+              // channels >= nc of a partial chunk hold stale LDS: select, not multiply by 0
 #pragma unroll
-              for (int c = 0; c < NCC; ++c) dot += bp[c * VOL + o] * tvv[c];
+              for (int c = 0; c < NCC; ++c) dot += c < nc ? bp[c * VOL + o] * tvv[c] : 0.f;
               og[k][0] += gx * dot;
               og[k][1] += gy * dot;
               og[k][2] += gz * dot;

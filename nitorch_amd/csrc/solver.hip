@@ -46,7 +46,7 @@
 namespace ni_amd {
 
 #define SV_BLOCK 256
-#define SV_SMALL 4096           // levels up to this many voxels (x batch): single-block relax
+#define SV_SMALL_BYTES 49152    // levels whose solution (3 N V values) fits: single-block relax in LDS
 
 struct StencilW {
   double vx[3];        // 1 / voxel_size^2
@@ -108,23 +108,31 @@ struct Geo {
 // Gather the 25 stencil positions of the three components around (x, y, z)
 // (radius-2 axial, radius-1 diagonals) and return (L u) for each component.
 // u: pointer to component 0 of this batch element; V = X*Y*Z.
-template <typename T>
-__device__ __forceinline__ void apply_L(const T *__restrict__ u, int V, const Geo &G,
-                                        const StencilT<T> &S, int x, int y, int z, T out[3]) {
+// INTERIOR: every position of the stencil is inside the volume (no folding,
+// all signs +1), which a wave checks before taking this path.
+template <typename T, bool INTERIOR>
+__device__ __forceinline__ void apply_L_t(const T *__restrict__ u, int V, const Geo &G,
+                                          const StencilT<T> &S, int x, int y, int z, T out[3]) {
   // per-axis folded offsets and signs for displacements -2..2
   int ox[5], oy[5], oz[5];
   int sx[5], sy[5], sz[5];
   const int sYZ = G.Y * G.Z, sZ = G.Z;
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
-    ox[k] = fold<int>(G.b[0], x + k - 2, G.X, sx[k]) * sYZ;
-    oy[k] = fold<int>(G.b[1], y + k - 2, G.Y, sy[k]) * sZ;
-    oz[k] = fold<int>(G.b[2], z + k - 2, G.Z, sz[k]);
+    if (INTERIOR) {
+      ox[k] = (x + k - 2) * sYZ; oy[k] = (y + k - 2) * sZ; oz[k] = z + k - 2;
+      sx[k] = sy[k] = sz[k] = 1;
+    } else {
+      ox[k] = fold<int>(G.b[0], x + k - 2, G.X, sx[k]) * sYZ;
+      oy[k] = fold<int>(G.b[1], y + k - 2, G.Y, sy[k]) * sZ;
+      oz[k] = fold<int>(G.b[2], z + k - 2, G.Z, sz[k]);
+    }
   }
   // value of component c at displacement (i, j, k) in {-2..2}, with sign
   auto at = [&](int c, int i, int j, int k) -> T {
-    const int s = sx[i + 2] * sy[j + 2] * sz[k + 2];
     const T v = u[c * V + ox[i + 2] + oy[j + 2] + oz[k + 2]];
+    if (INTERIOR) return v;
+    const int s = sx[i + 2] * sy[j + 2] * sz[k + 2];
     return s == 0 ? T(0) : (s < 0 ? -v : v);
   };
   T val[3][13];   // per component: c, x-1, x+1, y-1, y+1, z-1, z+1, x-2, x+2, y-2, y+2, z-2, z+2
@@ -167,6 +175,18 @@ __device__ __forceinline__ void apply_L(const T *__restrict__ u, int V, const Ge
   }
 }
 
+// (L u) at (x, y, z): the fold-free path when the whole wave is at least 2
+// voxels from every face, else the folding path (uniform per wave).
+template <typename T>
+__device__ __forceinline__ void apply_L(const T *__restrict__ u, int V, const Geo &G,
+                                        const StencilT<T> &S, int x, int y, int z, T out[3]) {
+  const bool in = x >= 2 && x < G.X - 2 && y >= 2 && y < G.Y - 2 && z >= 2 && z < G.Z - 2;
+  if (__all(in))
+    apply_L_t<T, true>(u, V, G, S, x, y, z, out);
+  else
+    apply_L_t<T, false>(u, V, G, S, x, y, z, out);
+}
+
 template <typename T>
 __device__ __forceinline__ void load_H(const T *__restrict__ h, int CC, int V, int i, T M[6]) {
   M[0] = M[1] = M[2] = M[3] = M[4] = M[5] = T(0);
@@ -185,7 +205,7 @@ template <typename T>
 __global__ __launch_bounds__(SV_BLOCK) void reg_kernel(const T *__restrict__ x, const T *__restrict__ h, int CC,
                                                        const T *__restrict__ g, T *__restrict__ out,
                                                        Geo G, StencilT<T> S, int N, int mode,
-                                                       double *__restrict__ acc) {
+                                                       double *__restrict__ acc, int nslot) {
   const int V = G.X * G.Y * G.Z;
   const long long gid = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
   double r2 = 0.0, g2 = 0.0;
@@ -230,8 +250,10 @@ __global__ __launch_bounds__(SV_BLOCK) void reg_kernel(const T *__restrict__ x, 
     if (threadIdx.x == 0) {
       double a = 0, b = 0;
       for (int k = 0; k < SV_BLOCK / 64; ++k) { a += s_r[k]; b += s_g[k]; }
-      atomicAdd(acc, a);
-      atomicAdd(acc + 1, b);
+      // acc = [2][nslot]: ||r||^2 and ||g||^2 partials (nslot = 1: the two sums)
+      const int slot = blockIdx.x % nslot;
+      atomicAdd(acc + slot, a);
+      atomicAdd(acc + nslot + slot, b);
     }
   }
 }
@@ -313,11 +335,18 @@ __global__ __launch_bounds__(SV_BLOCK) void relax_colour(T *__restrict__ x, cons
 
 // All sweeps of a small level in one work-group, with a barrier between
 // classes; threads enumerate the members of each class directly.
+// Small levels: one work-group runs every sweep and colour with barriers in
+// between, on a copy of the solution held in LDS (3 N V values, at most
+// SV_SMALL_BYTES), so the 75 stencil reads per voxel and the updates never
+// leave the CU.  32-bit index arithmetic (these levels are small).
 template <typename T>
 __global__ __launch_bounds__(1024) void relax_small(T *__restrict__ x, const T *__restrict__ g, const T *__restrict__ h,
                                                     int CC, Geo G, StencilT<T> S, int N, ColourPlan P, int nb_iter) {
-  // 32-bit index arithmetic: levels relaxed here hold at most SV_SMALL voxels
+  extern __shared__ char sv_smem[];
+  T *xs = (T *)sv_smem;
   const int V = G.X * G.Y * G.Z;
+  for (int i = threadIdx.x; i < 3 * N * V; i += blockDim.x) xs[i] = x[i];
+  __syncthreads();
   for (int it = 0; it < nb_iter; ++it) {
     if (P.redblack) {
       for (int rb = 0; rb < 2; ++rb) {
@@ -328,7 +357,7 @@ __global__ __launch_bounds__(1024) void relax_small(T *__restrict__ x, const T *
           const int row = j / nzh, y = row % G.Y, xx = row / G.Y;
           const int z = (j - row * nzh) * 2 + ((xx + y + rb) & 1);
           if (z < G.Z)
-            relax_voxel<T>(x + (size_t)n * 3 * V, g + (size_t)n * 3 * V, h ? h + (size_t)n * CC * V : nullptr,
+            relax_voxel<T>(xs + (size_t)n * 3 * V, g + (size_t)n * 3 * V, h ? h + (size_t)n * CC * V : nullptr,
                            CC, V, G, S, xx, y, z);
         }
         __syncthreads();
@@ -346,12 +375,13 @@ __global__ __launch_bounds__(1024) void relax_small(T *__restrict__ x, const T *
             const int z = P.ax[2].start[c] + P.ax[2].step[c] * jz;
             const int y = P.ax[1].start[b] + P.ax[1].step[b] * jy;
             const int xx = P.ax[0].start[a] + P.ax[0].step[a] * jx;
-            relax_voxel<T>(x + (size_t)n * 3 * V, g + (size_t)n * 3 * V, h ? h + (size_t)n * CC * V : nullptr,
+            relax_voxel<T>(xs + (size_t)n * 3 * V, g + (size_t)n * 3 * V, h ? h + (size_t)n * CC * V : nullptr,
                            CC, V, G, S, xx, y, z);
           }
           __syncthreads();
         }
   }
+  for (int i = threadIdx.x; i < 3 * N * V; i += blockDim.x) x[i] = xs[i];
 }
 
 // --------------------------------------------------------------------------
@@ -377,13 +407,18 @@ __device__ __forceinline__ void block_jacobi(const T *__restrict__ hn, int CC, i
   s2 = idt * (v0 * (p * s - q * b) + v1 * (p * q - a * s) + v2 * (a * b - p * p));
 }
 
-// Per-batch double reduction of one value per thread into out[n].  A block
-// spans at most two batch elements when V >= SV_BLOCK (both are reduced in
-// the block, one atomic each); smaller levels fall back to per-thread atomics.
-// Summation order across blocks is that of the atomics (double precision).
+// Per-batch double reductions.  Each scalar is held as SV_SLOTS partial
+// sums: a block adds into slot (block id % SV_SLOTS) of its batch element,
+// so thousands of blocks do not serialise on one address; consumers sum the
+// slots with one wave.  A block spans at most two batch elements when
+// V >= SV_BLOCK (both are reduced in the block); smaller levels fall back to
+// per-thread atomics.  Summation order is that of the atomics (FP64).
+#define SV_SLOTS 256
+
 __device__ __forceinline__ void batch_reduce(double acc, int n, int V, double *__restrict__ out) {
+  const int slot = blockIdx.x & (SV_SLOTS - 1);
   if (V < SV_BLOCK) {
-    if (acc != 0) atomicAdd(out + n, acc);
+    if (acc != 0) atomicAdd(out + (size_t)n * SV_SLOTS + slot, acc);
     return;
   }
   __shared__ double s_a[SV_BLOCK / 64], s_b[SV_BLOCK / 64];
@@ -398,9 +433,45 @@ __device__ __forceinline__ void batch_reduce(double acc, int n, int V, double *_
   if (threadIdx.x == 0) {
     a = 0; b = 0;
     for (int k = 0; k < SV_BLOCK / 64; ++k) { a += s_a[k]; b += s_b[k]; }
-    if (a != 0) atomicAdd(out + n0, a);
-    if (b != 0) atomicAdd(out + n0 + 1, b);
+    if (a != 0) atomicAdd(out + (size_t)n0 * SV_SLOTS + slot, a);
+    if (b != 0) atomicAdd(out + (size_t)(n0 + 1) * SV_SLOTS + slot, b);
   }
+}
+
+// Sum of the SV_SLOTS partials of scalar `q` for batch element n (called by
+// one full wave; the result is valid in every lane).
+__device__ __forceinline__ double slot_sum(const double *__restrict__ q, int n) {
+  const int lane = threadIdx.x & 63;
+  const double *p = q + (size_t)n * SV_SLOTS;
+  double v = 0;
+#pragma unroll
+  for (int j = 0; j < SV_SLOTS / 64; ++j) v += p[lane + 64 * j];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// The two scalars a/b of the (at most two) batch elements a block touches,
+// first and last element covered by the block; wave 0 reads, LDS broadcasts.
+struct BlockScalars {
+  double a[2], b[2];
+  int n0;
+};
+__device__ __forceinline__ void block_scalars(const double *__restrict__ qa, const double *__restrict__ qb,
+                                              int V, int N, long long per_block, BlockScalars &out) {
+  __shared__ double s_v[4];
+  const long long first = (long long)blockIdx.x * per_block;
+  const int n0 = (int)(first / V);
+  int n1 = (int)((first + per_block - 1) / V);
+  n1 = n1 < N ? n1 : N - 1;
+  if (threadIdx.x < 64) {
+    const double a0 = slot_sum(qa, n0), b0 = slot_sum(qb, n0);
+    const double a1 = n1 != n0 ? slot_sum(qa, n1) : a0, b1 = n1 != n0 ? slot_sum(qb, n1) : b0;
+    if (threadIdx.x == 0) { s_v[0] = a0; s_v[1] = b0; s_v[2] = a1; s_v[3] = b1; }
+  }
+  __syncthreads();
+  out.a[0] = s_v[0]; out.b[0] = s_v[1]; out.a[1] = s_v[2]; out.b[1] = s_v[3];
+  out.n0 = n0;
 }
 
 // r = g - (H + L) x ; z = M \ r ; p = z ; rz[n] += r.z
@@ -474,14 +545,17 @@ __global__ __launch_bounds__(SV_BLOCK) void pcg_update(T *__restrict__ x, T *__r
                                                        const double *__restrict__ pap, double *__restrict__ rz_new) {
   const int V = G.X * G.Y * G.Z;
   const long long gid = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
+  BlockScalars bs;
+  block_scalars(rz, pap, V, N, SV_BLOCK, bs);
   double acc = 0;
   int n = 0;
   if (gid < (long long)N * V) {
     n = (int)(gid / V);
     const int i = (int)(gid % V);
     const size_t b3 = (size_t)n * 3 * V;
-    const double den = pap[n] > 1e-12 ? pap[n] : 1e-12;
-    const T alpha = T(rz[n] / den);
+    const int j = n == bs.n0 ? 0 : 1;
+    const double den = bs.b[j] > 1e-12 ? bs.b[j] : 1e-12;
+    const T alpha = T(bs.a[j] / den);
     T rr[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -499,18 +573,24 @@ __global__ __launch_bounds__(SV_BLOCK) void pcg_update(T *__restrict__ x, T *__r
 }
 
 // beta = rz_new / max(rz, 1e-12); p = z + beta p.  Also clears the two
-// accumulators the next iteration adds into (neither is read here).
+// slot arrays the next iteration adds into (neither is read here).
 template <typename T>
 __global__ __launch_bounds__(SV_BLOCK) void pcg_direction(T *__restrict__ p, const T *__restrict__ z, int V, int N,
                                                           const double *__restrict__ rz,
                                                           const double *__restrict__ rz_new,
                                                           double *__restrict__ clear_a, double *__restrict__ clear_b) {
   const long long gid = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
-  if (gid < N) { clear_a[gid] = 0; clear_b[gid] = 0; }
+  for (long long k = gid; k < (long long)N * SV_SLOTS; k += (long long)gridDim.x * SV_BLOCK) {
+    clear_a[k] = 0;
+    clear_b[k] = 0;
+  }
+  BlockScalars bs;
+  block_scalars(rz_new, rz, 3 * V, N, SV_BLOCK, bs);
   if (gid >= (long long)N * 3 * V) return;
   const int n = (int)(gid / (3LL * V));
-  const double den = rz[n] > 1e-12 ? rz[n] : 1e-12;
-  const T beta = T(rz_new[n] / den);
+  const int j = n == bs.n0 ? 0 : 1;
+  const double den = bs.b[j] > 1e-12 ? bs.b[j] : 1e-12;
+  const T beta = T(bs.a[j] / den);
   p[gid] = z[gid] + beta * p[gid];
 }
 
@@ -624,9 +704,10 @@ static void launch_fill(T *p, long long n, T v, hipStream_t st) {
 
 template <typename T>
 static void launch_reg(const T *x, const T *h, int CC, const T *g, T *out, const Geo &G,
-                       const StencilT<T> &S, int N, int mode, double *acc, hipStream_t st) {
+                       const StencilT<T> &S, int N, int mode, double *acc, hipStream_t st, int nslot = 1) {
   const long long n = (long long)N * G.X * G.Y * G.Z;
-  hipLaunchKernelGGL(reg_kernel<T>, dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, x, h, CC, g, out, G, S, N, mode, acc);
+  hipLaunchKernelGGL(reg_kernel<T>, dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, x, h, CC, g, out, G, S, N, mode, acc,
+                     nslot);
 }
 
 static int relax_band(const double *pen) {
@@ -671,8 +752,9 @@ static void launch_relax(T *x, const T *g, const T *h, int CC, const Geo &G, con
                          int N, int band, int nb_iter, hipStream_t st) {
   const long long V = (long long)G.X * G.Y * G.Z;
   const ColourPlan P = make_plan(G, band);
-  if (V * N <= SV_SMALL) {
-    hipLaunchKernelGGL(relax_small<T>, dim3(1), dim3(1024), 0, st, x, g, h, CC, G, S, N, P, nb_iter);
+  const size_t xbytes = (size_t)3 * N * V * sizeof(T);
+  if (xbytes <= SV_SMALL_BYTES) {
+    hipLaunchKernelGGL(relax_small<T>, dim3(1), dim3(1024), xbytes, st, x, g, h, CC, G, S, N, P, nb_iter);
     return;
   }
   for (int it = 0; it < nb_iter; ++it) {
@@ -716,7 +798,8 @@ static void launch_transfer(const T *in, T *out, int N, int C, const int *si, co
     hipLaunchKernelGGL((transfer_kernel<T, 3>), dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, in, out, N * C, C, P, accumulate);
 }
 
-// PCG scratch: r, z, p (3 fields each) then 4 N doubles (rz ring of three, p.Ap)
+// PCG scratch: r, z, p (3 fields each) then 4 N SV_SLOTS doubles (slots of the
+// rz ring of three and of p.Ap)
 static size_t pcg_field_elems(int N, size_t V) { return (size_t)N * 3 * V; }
 static size_t align_up(size_t b, size_t a) { return (b + a - 1) / a * a; }
 
@@ -727,23 +810,28 @@ template <typename T>
 static int run_pcg(T *x, const T *g, const T *h, int CC, const Geo &G, const StencilT<T> &S, int N, int nb_iter,
                    double tol, T *r, T *z, T *p, double *sc, hipStream_t st, int *iters) {
   const long long V = (long long)G.X * G.Y * G.Z;
-  double *R[3] = {sc, sc + N, sc + 2 * N};
-  double *pap = sc + 3 * N;
-  launch_fill<double>(sc, 4LL * N, 0.0, st);
+  const size_t q = (size_t)N * SV_SLOTS;
+  double *R[3] = {sc, sc + q, sc + 2 * q};
+  double *pap = sc + 3 * q;
+  launch_fill<double>(sc, 4LL * (long long)q, 0.0, st);
   const unsigned bv = nblocks((long long)N * V), bf = nblocks((long long)N * 3 * V);
   hipLaunchKernelGGL(pcg_init<T>, dim3(bv), dim3(SV_BLOCK), 0, st, x, g, h, CC, r, z, p, G, S, N, R[0]);
   int it = 0;
-  std::vector<double> host(2 * (size_t)N);
+  std::vector<double> host(2 * q);
   for (; it < nb_iter; ++it) {
     double *cur = R[it % 3], *nxt = R[(it + 1) % 3], *spare = R[(it + 2) % 3];
     hipLaunchKernelGGL(pcg_matvec<T>, dim3(bv), dim3(SV_BLOCK), 0, st, p, h, CC, z, G, S, N, pap);
     if (tol > 0) {
-      NI_HIP_CHECK(hipMemcpyAsync(host.data(), pap, N * sizeof(double), hipMemcpyDeviceToHost, st));
-      NI_HIP_CHECK(hipMemcpyAsync(host.data() + N, cur, N * sizeof(double), hipMemcpyDeviceToHost, st));
+      NI_HIP_CHECK(hipMemcpyAsync(host.data(), pap, q * sizeof(double), hipMemcpyDeviceToHost, st));
+      NI_HIP_CHECK(hipMemcpyAsync(host.data() + q, cur, q * sizeof(double), hipMemcpyDeviceToHost, st));
       NI_HIP_CHECK(hipStreamSynchronize(st));
       double obj = 0;
       for (int n = 0; n < N; ++n) {
-        const double pa = host[n], rz = host[N + n];
+        double pa = 0, rz = 0;
+        for (int k = 0; k < SV_SLOTS; ++k) {
+          pa += host[(size_t)n * SV_SLOTS + k];
+          rz += host[q + (size_t)n * SV_SLOTS + k];
+        }
         const double alpha = rz / (pa > 1e-12 ? pa : 1e-12);
         obj += alpha * (alpha * pa + 2 * rz);
       }
@@ -789,10 +877,15 @@ static size_t fmg_ws_elems(int N, int CC, const Level *L, int NL) {
   return e;
 }
 
-// bytes: pyramid, then (PCG smoother) z and p at level-0 size and 4 N doubles
+// bytes: pyramid, the slots of the residual norms (2 SV_SLOTS doubles), then
+// (PCG smoother) z and p at level-0 size and 4 N SV_SLOTS doubles
+static size_t fmg_norms_offset(int N, int CC, const Level *L, int NL, size_t esz) {
+  return align_up(fmg_ws_elems(N, CC, L, NL) * esz, 256);
+}
 static size_t fmg_ws_bytes(int N, int CC, const Level *L, int NL, int use_cg, size_t esz) {
-  size_t b = fmg_ws_elems(N, CC, L, NL) * esz;
-  if (use_cg) b = align_up(b + 2 * pcg_field_elems(N, L[0].V) * esz, 256) + 4 * (size_t)N * sizeof(double);
+  size_t b = fmg_norms_offset(N, CC, L, NL, esz) + 2 * SV_SLOTS * sizeof(double);
+  if (use_cg)
+    b = align_up(b + 2 * pcg_field_elems(N, L[0].V) * esz, 256) + 4 * (size_t)N * SV_SLOTS * sizeof(double);
   return b;
 }
 
@@ -808,6 +901,7 @@ struct FMG {
   int bound[3];
   T *cg_z, *cg_p;          // PCG smoother scratch (sized for level 0)
   double *cg_sc;
+  double *norm_slots;      // 2 SV_SLOTS partials of ||r||^2, ||g||^2
   hipStream_t st;
 
   void factors(int l, float f[3]) const {  // fine / coarse size ratio between l and l+1
@@ -873,13 +967,15 @@ static int fmg_setup(FMG<T> &F, int N, const int *sz, const T *hess, int CC, con
   const size_t need = fmg_ws_bytes(N, F.CC, F.L, F.NL, use_cg, sizeof(T));
   NI_CHECK(ws_bytes >= need, "fmg_grid: workspace too small (%zu < %zu bytes)", ws_bytes, need);
   T *p = (T *)ws;
+  const size_t noff = fmg_norms_offset(N, F.CC, F.L, F.NL, sizeof(T));
+  F.norm_slots = (double *)((char *)ws + noff);
   F.cg_z = F.cg_p = nullptr;
   F.cg_sc = nullptr;
   if (use_cg) {
-    const size_t pyr = fmg_ws_elems(N, F.CC, F.L, F.NL);
-    F.cg_z = p + pyr;
+    const size_t zoff = noff + 2 * SV_SLOTS * sizeof(double);   // 256-aligned
+    F.cg_z = (T *)((char *)ws + zoff);
     F.cg_p = F.cg_z + pcg_field_elems(N, F.L[0].V);
-    F.cg_sc = (double *)((char *)ws + align_up((pyr + 2 * pcg_field_elems(N, F.L[0].V)) * sizeof(T), 256));
+    F.cg_sc = (double *)((char *)ws + align_up(zoff + 2 * pcg_field_elems(N, F.L[0].V) * sizeof(T), 256));
   }
   F.g0 = grad;
   F.h[0] = hess;
@@ -1077,26 +1173,28 @@ int ni_fmg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hess
 int ni_fmg_solve(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
                  const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
                  const double *penalty, int32_t nb_cycles, int32_t nb_iter, int32_t max_levels,
-                 int32_t use_cg, double tol, int32_t max_vcycles, double *norms, void *workspace, int64_t workspace_bytes,
+                 int32_t use_cg, double tol, int32_t max_vcycles, void *workspace, int64_t workspace_bytes,
                  double *out_rel, int32_t *out_vcycles, void *stream) {
   set_error("");
-  NI_CHECK(gradient && solution && shape && bound && voxel_size && penalty && workspace && norms,
+  NI_CHECK(gradient && solution && shape && bound && voxel_size && penalty && workspace,
            "fmg_solve: missing argument");
   const int sz[3] = {(int)shape[0], (int)shape[1], (int)shape[2]};
   if (int rc = fmg_check((int)N, sz, hessian ? CC : 0, bound, nb_iter)) return rc;
   hipStream_t st = (hipStream_t)stream;
-  double host[2] = {0, 0};
+  std::vector<double> host(2 * SV_SLOTS);
   int cycles = 0;
   double rel = 0;
   auto loop = [&](auto &F) -> int {
     using T = typename std::remove_reference<decltype(*F.x[0])>::type;
     if (nb_cycles > 0) fmg_run(F, nb_cycles);   // else: V-cycles from the given solution
     while (true) {
-      launch_fill<double>(norms, 2, 0.0, st);
-      launch_reg<T>(F.x[0], F.h[0], F.CC, F.g0, F.r[0], F.G[0], F.S[0], F.N, 1, norms, st);
-      NI_HIP_CHECK(hipMemcpyAsync(host, norms, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+      launch_fill<double>(F.norm_slots, 2 * SV_SLOTS, 0.0, st);
+      launch_reg<T>(F.x[0], F.h[0], F.CC, F.g0, F.r[0], F.G[0], F.S[0], F.N, 1, F.norm_slots, st, SV_SLOTS);
+      NI_HIP_CHECK(hipMemcpyAsync(host.data(), F.norm_slots, 2 * SV_SLOTS * sizeof(double), hipMemcpyDeviceToHost, st));
       NI_HIP_CHECK(hipStreamSynchronize(st));
-      rel = std::sqrt(host[0] / (host[1] > 0 ? host[1] : 1e-300));
+      double r2 = 0, g2 = 0;
+      for (int k = 0; k < SV_SLOTS; ++k) { r2 += host[k]; g2 += host[SV_SLOTS + k]; }
+      rel = std::sqrt(r2 / (g2 > 0 ? g2 : 1e-300));
       if (rel <= tol || cycles >= max_vcycles) break;
       F.vcycle(0);
       ++cycles;
@@ -1127,7 +1225,7 @@ int ni_fmg_solve(int32_t dtype, int64_t N, const int64_t *shape, const void *hes
 int64_t ni_pcg_grid_workspace(int32_t dtype, int64_t N, const int64_t *shape) {
   const size_t V = (size_t)shape[0] * shape[1] * shape[2];
   return (int64_t)(align_up(3 * pcg_field_elems((int)N, V) * (dtype == NI_FLOAT64 ? 8 : 4), 256) +
-                   4 * (size_t)N * sizeof(double));
+                   4 * (size_t)N * SV_SLOTS * sizeof(double));
 }
 
 int ni_pcg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,

@@ -228,97 +228,116 @@ __global__ __launch_bounds__(UN_BLOCK) void bias_grad(const bf16 *__restrict__ d
 }
 
 // ------------------------------------------------------------------------
-// elementwise kernels (bf16, channels-last)
+// elementwise kernels (bf16, channels-last).  One thread per 8 consecutive
+// channels of one voxel (16-byte loads and stores); channel counts are
+// multiples of 8 and element counts fit 32 bits.
 // ------------------------------------------------------------------------
+__device__ __forceinline__ bf16x8 ld8v(const bf16 *p) { return *(const bf16x8 *)p; }
+__device__ __forceinline__ void st8v(bf16 *p, bf16x8 v) { *(bf16x8 *)p = v; }
+
 // dpre = dy * (y > 0 ? 1 : slope)
 __global__ void lrelu_bwd(const bf16 *__restrict__ dy, const bf16 *__restrict__ y, bf16 *__restrict__ out, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * UN_BLOCK + threadIdx.x;
+  const int64_t i = ((int64_t)blockIdx.x * UN_BLOCK + threadIdx.x) * 8;
   if (i >= n) return;
-  const float g = (float)dy[i];
-  out[i] = (bf16)((float)y[i] > 0.f ? g : UN_SLOPE * g);
+  const bf16x8 g = ld8v(dy + i), a = ld8v(y + i);
+  bf16x8 o;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) o[j] = (bf16)((float)a[j] > 0.f ? (float)g[j] : UN_SLOPE * (float)g[j]);
+  st8v(out + i, o);
 }
 
 // 2x2x2 average pool: x [N, D, H, W, C] -> y [N, D/2, H/2, W/2, C]
 __global__ void avgpool2_fwd(const bf16 *__restrict__ x, bf16 *__restrict__ y, int N, int D, int H, int W, int C) {
-  const int64_t i = (int64_t)blockIdx.x * UN_BLOCK + threadIdx.x;
-  const int d2 = D / 2, h2 = H / 2, w2 = W / 2;
-  const int64_t n_out = (int64_t)N * d2 * h2 * w2 * C;
-  if (i >= n_out) return;
-  int64_t t = i;
-  const int c = (int)(t % C); t /= C;
-  const int z = (int)(t % w2); t /= w2;
-  const int yy = (int)(t % h2); t /= h2;
-  const int xx = (int)(t % d2);
-  const int n = (int)(t / d2);
-  float s = 0.f;
+  const int d2 = D / 2, h2 = H / 2, w2 = W / 2, C8 = C / 8;
+  const int i = blockIdx.x * UN_BLOCK + threadIdx.x;
+  if (i >= N * d2 * h2 * w2 * C8) return;
+  int t = i;
+  const int c8 = t % C8; t /= C8;
+  const int z = t % w2; t /= w2;
+  const int yy = t % h2; t /= h2;
+  const int xx = t % d2;
+  const int n = t / d2;
+  float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int q = 0; q < 8; ++q) {
+    const bf16x8 v = ld8v(x + (((((size_t)n * D + 2 * xx + (q >> 2)) * H + 2 * yy + ((q >> 1) & 1)) * W + 2 * z + (q & 1)) * C + c8 * 8));
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int j = 0; j < 8; ++j) s[j] += (float)v[j];
+  }
+  bf16x8 o;
 #pragma unroll
-      for (int e = 0; e < 2; ++e)
-        s += (float)x[((((int64_t)n * D + 2 * xx + a) * H + 2 * yy + b) * W + 2 * z + e) * C + c];
-  y[i] = (bf16)(s * 0.125f);
+  for (int j = 0; j < 8; ++j) o[j] = (bf16)(s[j] * 0.125f);
+  st8v(y + (size_t)i * 8, o);
 }
 
 // dx[child] = dy[parent] / 8
 __global__ void avgpool2_bwd(const bf16 *__restrict__ dy, bf16 *__restrict__ dx, int N, int D, int H, int W, int C) {
-  const int64_t i = (int64_t)blockIdx.x * UN_BLOCK + threadIdx.x;
-  const int64_t n_in = (int64_t)N * D * H * W * C;
-  if (i >= n_in) return;
-  int64_t t = i;
-  const int c = (int)(t % C); t /= C;
-  const int z = (int)(t % W); t /= W;
-  const int yy = (int)(t % H); t /= H;
-  const int xx = (int)(t % D);
-  const int n = (int)(t / D);
-  dx[i] = (bf16)((float)dy[((((int64_t)n * (D / 2) + xx / 2) * (H / 2) + yy / 2) * (W / 2) + z / 2) * C + c] * 0.125f);
+  const int C8 = C / 8;
+  const int i = blockIdx.x * UN_BLOCK + threadIdx.x;
+  if (i >= N * D * H * W * C8) return;
+  int t = i;
+  const int c8 = t % C8; t /= C8;
+  const int z = t % W; t /= W;
+  const int yy = t % H; t /= H;
+  const int xx = t % D;
+  const int n = t / D;
+  const bf16x8 v = ld8v(dy + ((((size_t)n * (D / 2) + xx / 2) * (H / 2) + yy / 2) * (W / 2) + z / 2) * C + c8 * 8);
+  bf16x8 o;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) o[j] = (bf16)((float)v[j] * 0.125f);
+  st8v(dx + (size_t)i * 8, o);
 }
 
 // out [N, D, H, W, C1 + C2] = concat(nearest_up2(a [N, D/2, H/2, W/2, C1]), skip [N, D, H, W, C2])
 __global__ void up2_concat_fwd(const bf16 *__restrict__ a, const bf16 *__restrict__ skip, bf16 *__restrict__ out,
                                int N, int D, int H, int W, int C1, int C2) {
-  const int64_t i = (int64_t)blockIdx.x * UN_BLOCK + threadIdx.x;
-  const int C = C1 + C2;
-  const int64_t n_out = (int64_t)N * D * H * W * C;
-  if (i >= n_out) return;
-  int64_t t = i;
-  const int c = (int)(t % C); t /= C;
-  const int64_t p = t;
-  const int z = (int)(t % W); t /= W;
-  const int yy = (int)(t % H); t /= H;
-  const int xx = (int)(t % D);
-  const int n = (int)(t / D);
+  const int C8 = (C1 + C2) / 8;
+  const int i = blockIdx.x * UN_BLOCK + threadIdx.x;
+  if (i >= N * D * H * W * C8) return;
+  int t = i;
+  const int c8 = t % C8; t /= C8;
+  const int p = t;
+  const int z = t % W; t /= W;
+  const int yy = t % H; t /= H;
+  const int xx = t % D;
+  const int n = t / D;
+  const int c = c8 * 8;
+  bf16x8 v;
   if (c < C1)
-    out[i] = a[((((int64_t)n * (D / 2) + xx / 2) * (H / 2) + yy / 2) * (W / 2) + z / 2) * C1 + c];
+    v = ld8v(a + ((((size_t)n * (D / 2) + xx / 2) * (H / 2) + yy / 2) * (W / 2) + z / 2) * C1 + c);
   else
-    out[i] = skip[p * C2 + (c - C1)];
+    v = ld8v(skip + (size_t)p * C2 + (c - C1));
+  st8v(out + (size_t)i * 8, v);
 }
 
 // backward of up2_concat_fwd: da = sum over the 8 children, dskip = slice
 __global__ void up2_concat_bwd(const bf16 *__restrict__ dout, bf16 *__restrict__ da, bf16 *__restrict__ dskip,
                                int N, int D, int H, int W, int C1, int C2) {
-  const int64_t i = (int64_t)blockIdx.x * UN_BLOCK + threadIdx.x;
-  const int C = C1 + C2;
-  const int d2 = D / 2, h2 = H / 2, w2 = W / 2;
-  const int64_t n_a = (int64_t)N * d2 * h2 * w2 * C1;
-  const int64_t n_s = (int64_t)N * D * H * W * C2;
-  if (i < n_a) {
-    int64_t t = i;
-    const int c = (int)(t % C1); t /= C1;
-    const int z = (int)(t % w2); t /= w2;
-    const int yy = (int)(t % h2); t /= h2;
-    const int xx = (int)(t % d2);
-    const int n = (int)(t / d2);
-    float s = 0.f;
+  const int C = C1 + C2, d2 = D / 2, h2 = H / 2, w2 = W / 2;
+  const int na = N * d2 * h2 * w2 * (C1 / 8), ns = N * D * H * W * (C2 / 8);
+  const int i = blockIdx.x * UN_BLOCK + threadIdx.x;
+  if (i < na) {
+    int t = i;
+    const int c8 = t % (C1 / 8); t /= C1 / 8;
+    const int z = t % w2; t /= w2;
+    const int yy = t % h2; t /= h2;
+    const int xx = t % d2;
+    const int n = t / d2;
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-    for (int q = 0; q < 8; ++q)
-      s += (float)dout[((((int64_t)n * D + 2 * xx + (q >> 2)) * H + 2 * yy + ((q >> 1) & 1)) * W + 2 * z + (q & 1)) * C + c];
-    da[i] = (bf16)s;
-  } else if (i < n_a + n_s) {
-    const int64_t j = i - n_a;
-    const int c = (int)(j % C2);
-    dskip[j] = dout[(j / C2) * C + C1 + c];
+    for (int q = 0; q < 8; ++q) {
+      const bf16x8 v = ld8v(dout + (((((size_t)n * D + 2 * xx + (q >> 2)) * H + 2 * yy + ((q >> 1) & 1)) * W + 2 * z + (q & 1)) * C + c8 * 8));
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s[j] += (float)v[j];
+    }
+    bf16x8 o;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o[j] = (bf16)s[j];
+    st8v(da + (size_t)i * 8, o);
+  } else if (i < na + ns) {
+    const int j = i - na;
+    const int c8 = j % (C2 / 8), p = j / (C2 / 8);
+    st8v(dskip + (size_t)j * 8, ld8v(dout + (size_t)p * C + C1 + c8 * 8));
   }
 }
 
@@ -457,8 +476,9 @@ int ni_unet_conv3_bwd_weight(int64_t N, const int64_t *shape, int32_t CI, int32_
 int ni_unet_lrelu_bwd(int64_t n, const void *dy, const void *y, void *out, void *stream) {
   set_error("");
   NI_CHECK(dy && y && out, "unet_lrelu_bwd: missing argument");
+  NI_CHECK(n % 8 == 0, "unet_lrelu_bwd: element count must be a multiple of 8");
   if (n > 0)
-    hipLaunchKernelGGL(lrelu_bwd, dim3(nblk(n)), dim3(UN_BLOCK), 0, (hipStream_t)stream, (const bf16 *)dy,
+    hipLaunchKernelGGL(lrelu_bwd, dim3(nblk(n / 8)), dim3(UN_BLOCK), 0, (hipStream_t)stream, (const bf16 *)dy,
                        (const bf16 *)y, (bf16 *)out, n);
   NI_HIP_CHECK(hipGetLastError());
   return 0;
@@ -470,12 +490,13 @@ int ni_unet_avgpool2(int32_t backward, int64_t N, const int64_t *shape, int32_t 
   NI_CHECK(in && out && shape, "unet_avgpool2: missing argument");
   const int D = (int)shape[0], H = (int)shape[1], W = (int)shape[2];
   NI_CHECK(D % 2 == 0 && H % 2 == 0 && W % 2 == 0, "unet_avgpool2: even sizes required");
+  NI_CHECK(C % 8 == 0 && N * D * H * W * C < (1LL << 31), "unet_avgpool2: C % 8 == 0 and < 2^31 elements required");
   hipStream_t st = (hipStream_t)stream;
   if (!backward)
-    hipLaunchKernelGGL(avgpool2_fwd, dim3(nblk(N * D * H * W * C / 8)), dim3(UN_BLOCK), 0, st, (const bf16 *)in,
+    hipLaunchKernelGGL(avgpool2_fwd, dim3(nblk(N * D * H * W * C / 64)), dim3(UN_BLOCK), 0, st, (const bf16 *)in,
                        (bf16 *)out, (int)N, D, H, W, C);
   else
-    hipLaunchKernelGGL(avgpool2_bwd, dim3(nblk(N * D * H * W * C)), dim3(UN_BLOCK), 0, st, (const bf16 *)in,
+    hipLaunchKernelGGL(avgpool2_bwd, dim3(nblk(N * D * H * W * C / 8)), dim3(UN_BLOCK), 0, st, (const bf16 *)in,
                        (bf16 *)out, (int)N, D, H, W, C);
   NI_HIP_CHECK(hipGetLastError());
   return 0;
@@ -487,13 +508,15 @@ int ni_unet_up2_concat(int32_t backward, int64_t N, const int64_t *shape, int32_
   NI_CHECK(a && skip && out && shape, "unet_up2_concat: missing argument");
   const int D = (int)shape[0], H = (int)shape[1], W = (int)shape[2];
   NI_CHECK(D % 2 == 0 && H % 2 == 0 && W % 2 == 0, "unet_up2_concat: even sizes required");
+  NI_CHECK(C1 % 8 == 0 && C2 % 8 == 0 && N * D * H * W * (C1 + C2) < (1LL << 31),
+           "unet_up2_concat: channel counts multiple of 8 and < 2^31 elements required");
   hipStream_t st = (hipStream_t)stream;
   if (!backward)
-    hipLaunchKernelGGL(up2_concat_fwd, dim3(nblk(N * D * H * W * (C1 + C2))), dim3(UN_BLOCK), 0, st, (const bf16 *)a,
-                       (const bf16 *)skip, (bf16 *)out, (int)N, D, H, W, C1, C2);
+    hipLaunchKernelGGL(up2_concat_fwd, dim3(nblk(N * D * H * W * (C1 + C2) / 8)), dim3(UN_BLOCK), 0, st,
+                       (const bf16 *)a, (const bf16 *)skip, (bf16 *)out, (int)N, D, H, W, C1, C2);
   else  // a <- d(low-res input), skip <- d(skip), out = incoming gradient
-    hipLaunchKernelGGL(up2_concat_bwd, dim3(nblk(N * D * H * W * C1 / 8 + N * D * H * W * C2)), dim3(UN_BLOCK), 0,
-                       st, (const bf16 *)out, (bf16 *)a, (bf16 *)skip, (int)N, D, H, W, C1, C2);
+    hipLaunchKernelGGL(up2_concat_bwd, dim3(nblk(N * D * H * W * C1 / 64 + N * D * H * W * C2 / 8)), dim3(UN_BLOCK),
+                       0, st, (const bf16 *)out, (bf16 *)a, (bf16 *)skip, (int)N, D, H, W, C1, C2);
   NI_HIP_CHECK(hipGetLastError());
   return 0;
 }

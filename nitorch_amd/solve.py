@@ -31,7 +31,7 @@ L.register_signatures({
                     L._I32, L._I32, L._I32, L._I32, L._I32, L._P, L._I64, L._P],
     "ni_fmg_solve": [L._I32, L._I64, L._PI64, L._P, L._I32, L._P, L._P, L._PI32,
                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
-                     L._I32, L._I32, L._I32, L._I32, ctypes.c_double, L._I32, L._P, L._P, L._I64,
+                     L._I32, L._I32, L._I32, L._I32, ctypes.c_double, L._I32, L._P, L._I64,
                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32), L._P],
     "ni_pcg_grid": [L._I32, L._I64, L._PI64, L._P, L._I32, L._P, L._P, L._PI32,
                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
@@ -359,7 +359,7 @@ def solve_grid_closedform(hessian, gradient, absolute=0, factor=1, voxel_size=1)
 
 def solve_to_tolerance(hessian, gradient, penalty, voxel_size=(1, 1, 1), bound="dft",
                        tol=1e-4, nb_cycles=2, nb_iter=4, max_vcycles=20, max_levels=16, x=None,
-                       workspace=None, norms=None, use_cg=False):
+                       workspace=None, use_cg=False):
     """Plain GPU FMG followed by V-cycles on the finest level until
     ||g - (H + L) x|| / ||g|| <= tol.  Channel-first contiguous tensors.
     Returns (x, relative residual, number of extra V-cycles)."""
@@ -370,13 +370,11 @@ def solve_to_tolerance(hessian, gradient, penalty, voxel_size=(1, 1, 1), bound="
     nbytes = _ws_fn()(dt, g.shape[0], L.i64arr(g.shape[2:]), CC, int(max_levels), int(bool(use_cg)))
     if workspace is None or workspace.numel() < nbytes:
         workspace = torch.empty(int(nbytes), dtype=torch.uint8, device=g.device)
-    if norms is None:
-        norms = torch.zeros(2, dtype=torch.float64, device=g.device)
     rel = ctypes.c_double(0)
     ncyc = ctypes.c_int32(0)
     L.check(L.lib().ni_fmg_solve(
         dt, g.shape[0], L.i64arr(g.shape[2:]), L.ptr(h), CC, L.ptr(g), L.ptr(x), _bounds(bound),
         _voxel(voxel_size), _dvec(penalty), int(nb_cycles), int(nb_iter), int(max_levels),
-        int(bool(use_cg)), float(tol), int(max_vcycles), L.ptr(norms), L.ptr(workspace), int(nbytes),
+        int(bool(use_cg)), float(tol), int(max_vcycles), L.ptr(workspace), int(nbytes),
         ctypes.byref(rel), ctypes.byref(ncyc), L.stream_of(g)), "fmg_solve")
     return x, rel.value, ncyc.value
