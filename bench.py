@@ -1,6 +1,6 @@
 """Benchmark of the spline push/pull hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-sweep] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-sweep] [--no-cpu] [--no-solve] [--copy-calibration]
 
 Workload ("pushpull_192"): input 4x8x192^3 float32 U[0,1), grid = identity +
 smoothed displacement with 5% of the points out of bounds (seeded,
@@ -40,6 +40,9 @@ def parse():
     p.add_argument("--cpu-slices", type=int, default=192,
                    help="CPU baseline sample: first K x-slices of every batch element")
     p.add_argument("--no-solve", action="store_true")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="first copy 1 GiB device to device (a known-size read and write for "
+                        "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return p.parse_args()
 
 
@@ -70,6 +73,13 @@ def main():
 
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
+    if args.copy_calibration:
+        a = torch.ones(256 << 20, device=dev)      # 1 GiB
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)                                 # reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        del a, b
     x_cpu, g_cpu = config2_inputs()
     x = x_cpu.to(dev)
     g = g_cpu.to(dev)
@@ -163,7 +173,7 @@ def main():
                    "samples_per_op": S, "oob_fraction": round(oob_frac, 4)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": measured_traffic(dom),
                      "algorithmic_bytes_per_launch": int(bytes_per_sample * S)},
         "ops": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in ops.items()},
         "sweep": sweep,
@@ -171,6 +181,19 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
+
+
+def measured_traffic(op):
+    """HBM bytes per launch of the op's kernels, from the committed counter
+    runs (profiles/traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE per
+    dispatch, divided by what the same counters read for a copy of known
+    size); None when no counter run is recorded for this op."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f).get(op)
+    return None if rec is None else int(rec["hbm_bytes_per_launch"])
 
 
 def gn_solve(dev):

@@ -48,6 +48,16 @@ namespace ni_amd {
 #define SV_BLOCK 256
 #define SV_SMALL_BYTES 49152    // levels whose solution (3 N V values) fits: single-block relax in LDS
 
+// Logical block id: the hardware dispatches consecutive blocks round-robin
+// over the 8 XCDs of an MI355X, each with its own L2; remapping gives every
+// XCD one contiguous range of blocks, i.e. a slab of x-planes, so the
+// +-2-plane neighbours of a stencil are mostly in the same L2.
+__device__ __forceinline__ long long sv_block() {
+  const long long nb = gridDim.x, b = blockIdx.x;
+  const long long xcd = b % 8, local = b / 8, q = nb / 8, r = nb % 8;
+  return xcd * q + (xcd < r ? xcd : r) + local;
+}
+
 struct StencilW {
   double vx[3];        // 1 / voxel_size^2
   double lam0;         // absolute
@@ -207,7 +217,7 @@ __global__ __launch_bounds__(SV_BLOCK) void reg_kernel(const T *__restrict__ x, 
                                                        Geo G, StencilT<T> S, int N, int mode,
                                                        double *__restrict__ acc, int nslot) {
   const int V = G.X * G.Y * G.Z;
-  const long long gid = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
+  const long long gid = sv_block() * SV_BLOCK + threadIdx.x;
   double r2 = 0.0, g2 = 0.0;
   if (gid < (long long)N * V) {
     const int n = (int)(gid / V), i = (int)(gid % V);
@@ -251,7 +261,7 @@ __global__ __launch_bounds__(SV_BLOCK) void reg_kernel(const T *__restrict__ x, 
       double a = 0, b = 0;
       for (int k = 0; k < SV_BLOCK / 64; ++k) { a += s_r[k]; b += s_g[k]; }
       // acc = [2][nslot]: ||r||^2 and ||g||^2 partials (nslot = 1: the two sums)
-      const int slot = blockIdx.x % nslot;
+      const int slot = (int)(sv_block() % nslot);
       atomicAdd(acc + slot, a);
       atomicAdd(acc + nslot + slot, b);
     }
@@ -306,7 +316,7 @@ __global__ __launch_bounds__(SV_BLOCK) void relax_colour(T *__restrict__ x, cons
                                                          int sx, int tx, int nx, int sy, int ty, int ny,
                                                          int sz, int tz, int nz) {
   const int V = G.X * G.Y * G.Z;
-  const long long gid = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
+  const long long gid = sv_block() * SV_BLOCK + threadIdx.x;
   int n, xx, y, z;
   if (!redblack) {
     const long long per = (long long)nx * ny * nz;
@@ -416,13 +426,13 @@ __device__ __forceinline__ void block_jacobi(const T *__restrict__ hn, int CC, i
 #define SV_SLOTS 256
 
 __device__ __forceinline__ void batch_reduce(double acc, int n, int V, double *__restrict__ out) {
-  const int slot = blockIdx.x & (SV_SLOTS - 1);
+  const int slot = (int)(sv_block() & (SV_SLOTS - 1));
   if (V < SV_BLOCK) {
     if (acc != 0) atomicAdd(out + (size_t)n * SV_SLOTS + slot, acc);
     return;
   }
   __shared__ double s_a[SV_BLOCK / 64], s_b[SV_BLOCK / 64];
-  const int n0 = (int)(((long long)blockIdx.x * SV_BLOCK) / V);
+  const int n0 = (int)((sv_block() * SV_BLOCK) / V);
   double a = n == n0 ? acc : 0.0, b = n == n0 ? 0.0 : acc;
   for (int off = 32; off > 0; off >>= 1) {
     a += __shfl_down(a, off, 64);
@@ -460,7 +470,7 @@ struct BlockScalars {
 __device__ __forceinline__ void block_scalars(const double *__restrict__ qa, const double *__restrict__ qb,
                                               int V, int N, long long per_block, BlockScalars &out) {
   __shared__ double s_v[4];
-  const long long first = (long long)blockIdx.x * per_block;
+  const long long first = sv_block() * per_block;
   const int n0 = (int)(first / V);
   int n1 = (int)((first + per_block - 1) / V);
   n1 = n1 < N ? n1 : N - 1;
@@ -481,7 +491,7 @@ __global__ __launch_bounds__(SV_BLOCK) void pcg_init(const T *__restrict__ x, co
                                                      T *__restrict__ z, T *__restrict__ p, Geo G, StencilT<T> S,
                                                      int N, double *__restrict__ rz) {
   const int V = G.X * G.Y * G.Z;
-  const long long gid = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
+  const long long gid = sv_block() * SV_BLOCK + threadIdx.x;
   double acc = 0;
   int n = 0;
   if (gid < (long long)N * V) {
@@ -514,7 +524,7 @@ __global__ __launch_bounds__(SV_BLOCK) void pcg_matvec(const T *__restrict__ p, 
                                                        T *__restrict__ Ap, Geo G, StencilT<T> S, int N,
                                                        double *__restrict__ pap) {
   const int V = G.X * G.Y * G.Z;
-  const long long gid = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
+  const long long gid = sv_block() * SV_BLOCK + threadIdx.x;
   double acc = 0;
   int n = 0;
   if (gid < (long long)N * V) {
@@ -544,7 +554,7 @@ __global__ __launch_bounds__(SV_BLOCK) void pcg_update(T *__restrict__ x, T *__r
                                                        Geo G, StencilT<T> S, int N, const double *__restrict__ rz,
                                                        const double *__restrict__ pap, double *__restrict__ rz_new) {
   const int V = G.X * G.Y * G.Z;
-  const long long gid = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
+  const long long gid = sv_block() * SV_BLOCK + threadIdx.x;
   BlockScalars bs;
   block_scalars(rz, pap, V, N, SV_BLOCK, bs);
   double acc = 0;
@@ -579,7 +589,7 @@ __global__ __launch_bounds__(SV_BLOCK) void pcg_direction(T *__restrict__ p, con
                                                           const double *__restrict__ rz,
                                                           const double *__restrict__ rz_new,
                                                           double *__restrict__ clear_a, double *__restrict__ clear_b) {
-  const long long gid = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
+  const long long gid = sv_block() * SV_BLOCK + threadIdx.x;
   for (long long k = gid; k < (long long)N * SV_SLOTS; k += (long long)gridDim.x * SV_BLOCK) {
     clear_a[k] = 0;
     clear_b[k] = 0;
@@ -598,7 +608,7 @@ __global__ __launch_bounds__(SV_BLOCK) void pcg_direction(T *__restrict__ p, con
 template <typename T>
 __global__ __launch_bounds__(SV_BLOCK) void precond_kernel(const T *__restrict__ g, const T *__restrict__ h, int CC,
                                                            T *__restrict__ out, int V, int N, StencilT<T> S) {
-  const long long gid = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
+  const long long gid = sv_block() * SV_BLOCK + threadIdx.x;
   if (gid >= (long long)N * V) return;
   const int n = (int)(gid / V), i = (int)(gid % V);
   const size_t b3 = (size_t)n * 3 * V;
@@ -633,7 +643,7 @@ template <typename T, int NT>
 __global__ __launch_bounds__(SV_BLOCK) void transfer_kernel(const T *__restrict__ in, T *__restrict__ out,
                                                             int NC, int C, Xfer P, int accumulate) {
   const int Vo = P.Xo * P.Yo * P.Zo, Vi = P.Xi * P.Yi * P.Zi;
-  const long long gid = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
+  const long long gid = sv_block() * SV_BLOCK + threadIdx.x;
   if (gid >= (long long)NC * Vo) return;
   const int nc = (int)(gid / Vo), o = (int)(gid % Vo);
   const int w[3] = {o / (P.Yo * P.Zo), (o / P.Zo) % P.Yo, o % P.Zo};

@@ -38,6 +38,15 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 #define UN_MB 4          // 16-position row blocks per wave
 #define UN_SLOPE 0.1f    // leaky ReLU negative slope
 
+// XCD-contiguous logical block id (the 8 XCDs of an MI355X each get one
+// contiguous range of blocks, so the +-1 x/y neighbour rows of a convolution
+// are mostly in the same L2).
+__device__ __forceinline__ int64_t un_block() {
+  const int64_t nb = gridDim.x, b = blockIdx.x;
+  const int64_t xcd = b % 8, local = b / 8, q = nb / 8, r = nb % 8;
+  return xcd * q + (xcd < r ? xcd : r) + local;
+}
+
 template <int CI>
 struct KPad {
   static constexpr int value = (27 * CI + 31) / 32 * 32;
@@ -66,7 +75,7 @@ __global__ __launch_bounds__(UN_BLOCK) void conv3_fwd(const bf16 *__restrict__ X
   const int64_t total = (int64_t)N * D * H * WB;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int row = lane & 15, ks = lane >> 4;
-  const int64_t pb0 = ((int64_t)blockIdx.x * (UN_BLOCK / 64) + wave) * UN_MB;
+  const int64_t pb0 = (un_block() * (UN_BLOCK / 64) + wave) * UN_MB;
 
   int px[UN_MB], py[UN_MB], pz[UN_MB], pn[UN_MB];
   bool pv[UN_MB];
