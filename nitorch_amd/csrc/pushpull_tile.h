@@ -1,18 +1,20 @@
-// LDS-tiled pull / push / count for 3-D float fields (isotropic order ORD).
+// LDS-tiled pull / push / count / gradients for 3-D float fields (isotropic
+// order ORD).
 //
-// One work-group owns a TL_X x TL_Y x TL_Z tile of samples (grid points).
-// It reduces the bounding box of the first taps of its in-bound samples,
-// and when the box (extended by the ORD + 1 taps) fits in LDS:
-//  * pull  : stages the box of the source -- folded by the boundary
-//            condition and signed, so the taps need no folding -- in LDS
-//            with coalesced loads, then gathers every tap from LDS;
-//  * push / count : accumulates the tile's splats into an LDS box with
+// One work-group owns a TX x TY x TZ tile of samples (grid points); each
+// thread owns R of them.  The work-group reduces the bounding box of the
+// first taps of its in-bound samples, and when the box (extended by the
+// ORD + 1 taps) fits in LDS:
+//  * gather modes (pull, gradients): stage the box of the source -- folded
+//            by the boundary condition and signed, so the taps need no
+//            folding -- in LDS, then read every tap from LDS;
+//  * push / count : accumulate the tile's splats into an LDS box with
 //            integer LDS atomics on a fixed-point scale chosen per tile and
-//            channel (2^k with  n_in * max|value| * 2^k < 2^30, so no sum can
+//            channel (2^e with  n_in * max|value| * 2^e < 2^30, so no sum can
 //            overflow; LDS float atomics are far slower than integer ones on
-//            gfx950), then adds the box to the output once, folded and
+//            gfx950), then add the box to the output once, folded and
 //            signed, with one FP32 global atomic per non-zero box element.
-//            Rounding: <= 2^-(21 - log2 n_in) * max|value| per splat, and the
+//            Rounding: <= 2^-(30 - log2 n_in) * max|value| per splat; the
 //            in-tile sum is exact and order-independent.
 // The box has fixed dimensions per order (every tap offset is then an
 // immediate of the LDS instruction); channels are staged NCC at a time.
@@ -21,10 +23,12 @@
 // direct per-sample routine pp_sample for its samples, so the result never
 // depends on the data's smoothness, only the speed does.
 //
-// Weights, tap order (z outer, x inner) and the out-of-bound rule are those
-// of pp_sample (nitorch/_C/src/impl/pushpull_common.cpp: OOB :926-974,
+// Weights, the out-of-bound rule and the splat order of each sample are
+// those of pp_sample (nitorch/_C/src/impl/pushpull_common.cpp: OOB :926-974,
 // interpolate3d / splat3d :1176-1290, linear :1653-1858, nearest
-// :2314-2341); folding and signs follow bounds_common.h:28-219.
+// :2314-2341); folding and signs follow bounds_common.h:28-219.  Gathers
+// sum the taps x-outer, z-inner (pp_sample: z-outer), which differs only by
+// rounding.
 #pragma once
 #include <climits>
 
@@ -32,24 +36,81 @@
 
 namespace ni_amd {
 
-#define TL_X 4
-#define TL_Y 8
-#define TL_Z 16
-#define TL_SAMPLES (TL_X * TL_Y * TL_Z)
-#define TL_PER_THREAD (TL_SAMPLES / NI_BLOCK)
 #define TL_SLACK 2     // extra box extent per axis for the spread of a tile's coordinates
 
-// Box geometry of order ORD, fixed at compile time so that every tap offset
-// is an immediate of the LDS instruction: the box covers the tile plus the
-// ORD + 1 taps plus TL_SLACK voxels per axis.
-template <int ORD>
+// Tile and box geometry of order ORD, fixed at compile time.
+//  * Gathers: 4 x 8 x 16 tiles; a wave holds 4 x-rows of 16 z-samples, and
+//    the two samples of a thread are 4 y-rows apart.
+//  * Scatters: larger tiles -- every box element is one global atomic in
+//    the flush, and a box exceeds its tile by ORD + 1 + TL_SLACK voxels per
+//    axis, so the atomics per sample fall with the tile size: 8 x 8 x 16
+//    (R = 4 samples per thread, consecutive along z) up to order 3, and
+//    4 x 8 x 16 (R = 2) above, where the weights need twice the registers.
+//  * The box covers the tile plus the ORD + 1 taps plus TL_SLACK voxels per
+//    axis.  Its x stride DYZP is 48 (mod 64) words and, for scatters, DZ is
+//    1 (mod R): the lanes of a wave reading or adding one tap then fall on
+//    distinct LDS banks (gathers: the four x-rows in the four quarters of
+//    the 64 banks; scatters: the R y-rows of a wave in the R residue
+//    classes mod R, its x-rows in disjoint slots of each class).
+template <int ORD, bool SCAT>
 struct TileBox {
-  static constexpr int DX = TL_X + ORD + TL_SLACK;
-  static constexpr int DY = TL_Y + ORD + TL_SLACK;
-  static constexpr int DZ = TL_Z + ORD + TL_SLACK;
+  static constexpr int R = SCAT && ORD <= 3 ? 4 : 2;   // samples per thread
+  static constexpr int TX = SCAT && ORD <= 3 ? 8 : 4;
+  static constexpr int TY = 8;
+  static constexpr int TZ = 16;
+  static constexpr int SDY = SCAT ? 0 : NI_BLOCK / 64;  // step between a thread's samples
+  static constexpr int SDZ = SCAT ? 1 : 0;
+  static constexpr int ZL = SCAT ? TZ / R : TZ;         // threads per z-row of the tile
+  static constexpr int RY = SCAT ? R : 1;               // y-rows per wave
+  static constexpr int RX = 64 / ZL / RY;               // x-rows per wave
+  static constexpr int WX = TX / RX;                    // waves along x
+  static constexpr int DX = TX + ORD + TL_SLACK;
+  static constexpr int DY = TY + ORD + TL_SLACK;
+  static constexpr int DZ0 = TZ + ORD + TL_SLACK;
+  static constexpr int DZ = SCAT ? DZ0 + ((1 - DZ0) % R + R) % R : DZ0;
   static constexpr int DYZ = DY * DZ;
-  static constexpr int VOL = DX * DYZ;
-  static constexpr int MAX_NCC = ORD <= 3 ? 4 : 2;   // channels staged at once (LDS budget)
+  static constexpr int DYZP = DYZ + ((48 - DYZ) % 64 + 64) % 64;
+  static constexpr int VOL = DX * DYZP;
+  static constexpr int NPP = (DYZ + NI_BLOCK - 1) / NI_BLOCK;   // y-z plane positions per thread
+  static constexpr int MAX_NCC = 2;                     // channels staged at once
+  static_assert(TX * TY * TZ == NI_BLOCK * R, "R samples per thread");
+  static_assert(WX * RX == TX, "waves tile the x rows");
+  static_assert(SCAT ? (NI_BLOCK / 64 / WX) * RY == TY : (NI_BLOCK / 64) * R == TY, "waves tile the y rows");
+};
+
+// First sample of this thread within its tile (sample i adds i * (SDY, SDZ)).
+template <class B>
+__device__ __forceinline__ void tl_thread(int &sx, int &sy, int &sz0) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int row = lane / B::ZL;
+  sz0 = (lane % B::ZL) * (B::SDZ ? B::R : 1);
+  sx = row % B::RX + B::RX * (w % B::WX);
+  sy = B::SDZ ? row / B::RX + B::RY * (w / B::WX) : w;
+}
+
+// The y-z plane of a box's ext[1] x ext[2] corner, split over the threads:
+// position j of a thread is (by, bz), with folded source offset and sign
+// from the axis tables and the LDS offset by * DZ + bz.
+template <class B>
+struct TilePlane {
+  int yz_ofs[B::NPP];   // folded y + z offset in the tensor, elements
+  int yz_lds[B::NPP];   // by * DZ + bz, or -1 when the position is outside the plane
+  float yz_sg[B::NPP];  // product of the y and z signs
+  __device__ __forceinline__ void init(int E1, int E2, const int *fofs, const float *fsg) {
+    const float r2 = 1.f / (float)E2;
+#pragma unroll
+    for (int j = 0; j < B::NPP; ++j) {
+      const int p = threadIdx.x + j * NI_BLOCK;
+      int by = (int)((float)p * r2);   // floor(p / E2): exact after the fix-ups (p < 2^22)
+      by -= by * E2 > p;
+      by += (by + 1) * E2 <= p;
+      const int bz = p - by * E2;
+      const bool in = by < E1;
+      yz_lds[j] = in ? by * B::DZ + bz : -1;
+      yz_ofs[j] = in ? fofs[B::DX + by] + fofs[B::DX + B::DY + bz] : 0;
+      yz_sg[j] = in ? fsg[B::DX + by] * fsg[B::DX + B::DY + bz] : 0.f;
+    }
+  }
 };
 
 // Gather modes stage the source box; scatter modes accumulate one.
@@ -66,6 +127,37 @@ struct TileMeta {
   static constexpr bool gather = MODE == TM_PULL || MODE == TM_SGRAD || MODE == TM_GGRAD || MODE == TM_GGRAD1;
   static constexpr bool deriv = MODE == TM_SGRAD || MODE == TM_GGRAD || MODE == TM_GGRAD1;
 };
+
+// Spline weight / derivative of tap T at signed distance d: the
+// evaluation of spl_w / spl_g (common.h), with the piece and its Horner
+// coefficients fixed at compile time (immediates, never table loads).
+template <int O, int P, int I>
+__device__ __forceinline__ float tl_horner_w(float u) {
+  constexpr float c = (float)SplTab<O>::W[P][I];
+  if constexpr (I == SplTab<O>::NDW - 1) return c;
+  else return tl_horner_w<O, P, I + 1>(u) * u + c;
+}
+template <int O, int P, int I>
+__device__ __forceinline__ float tl_horner_g(float u) {
+  constexpr float c = (float)SplTab<O>::G[P][I];
+  if constexpr (I == SplTab<O>::NDG - 1) return c;
+  else return tl_horner_g<O, P, I + 1>(u) * u + c;
+}
+template <int ORD, bool DERIV, int T>
+__device__ __forceinline__ void tl_weight(float x, int lo, float (&w)[ORD + 1], float (&g)[ORD + 1]) {
+  if constexpr (T <= ORD) {
+    constexpr int P = tap_piece<ORD>(T);
+    constexpr float centre = (float)piece_centre<ORD>(P);
+    const float d = x - (float)(lo + T);
+    const float a = d < 0.f ? -d : d;
+    w[T] = tl_horner_w<ORD, P, 0>(a - centre);
+    if constexpr (DERIV) {
+      const float gg = tl_horner_g<ORD, P, 0>(a - centre);
+      g[T] = d < 0.f ? -gg : gg;
+    }
+    tl_weight<ORD, DERIV, T + 1>(x, lo, w, g);
+  }
+}
 
 // First tap index and the ORD + 1 weights (and, when DERIV, first
 // derivatives) of coordinate x, as pp_sample computes them.
@@ -85,11 +177,7 @@ __device__ __forceinline__ int tl_taps(float x, float (&w)[ORD + 1], float (&g)[
     return j0;
   } else {
     const int lo = (int)floorf(x - first_tap_offset<float>(ORD));
-#pragma unroll
-    for (int t = 0; t <= ORD; ++t) {
-      w[t] = spl_w<float, ORD>(t, x - (float)(lo + t));
-      if (DERIV) g[t] = spl_g<float, ORD>(t, x - (float)(lo + t));
-    }
+    tl_weight<ORD, DERIV, 0>(x, lo, w, g);
     return lo;
   }
 }
@@ -105,6 +193,26 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
+// Rotate a small register array left by one: a rolled tap loop reads
+// element 0 and rotates, so that no array is indexed by the loop counter
+// (which would put it in scratch memory); NT rotations restore the order.
+template <typename T, int N>
+__device__ __forceinline__ void tl_rot(T (&v)[N]) {
+  const T t = v[0];
+#pragma unroll
+  for (int j = 0; j + 1 < N; ++j) v[j] = v[j + 1];
+  v[N - 1] = t;
+}
+
+// Hide a register array's values from the optimiser for one loop
+// iteration, so that products of it are not hoisted out of the loop (where
+// they would all be live at once).
+template <int N>
+__device__ __forceinline__ void tl_opaque(float (&v)[N]) {
+#pragma unroll
+  for (int j = 0; j < N; ++j) asm volatile("" : "+v"(v[j]));
+}
+
 template <int MODE>
 struct TileFlags {
   static constexpr int value = MODE == TM_PULL ? F_PULL : MODE == TM_PUSH ? (F_PUSH | F_TRGT)
@@ -118,8 +226,10 @@ template <int ORD, int MODE, int NCC>
 __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx, int nty, int ntz,
                                                         int64_t nblocks) {
   constexpr int NT = ORD + 1;
-  using B = TileBox<ORD>;
-  constexpr int DY = B::DY, DZ = B::DZ, DYZ = B::DYZ, VOL = B::VOL;
+  constexpr bool GATHER = TileMeta<MODE>::gather, DERIV = TileMeta<MODE>::deriv;
+  using B = TileBox<ORD, !GATHER>;
+  constexpr int R = B::R;
+  constexpr int DY = B::DY, DZ = B::DZ, DYZ = B::DYZP, VOL = B::VOL;
   static_assert(NCC >= 1 && NCC <= B::MAX_NCC, "channel chunk exceeds the LDS budget");
   __shared__ float box[NCC * VOL];
   __shared__ int red[NI_BLOCK / 64][6];
@@ -132,42 +242,48 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
   const int X = (int)a.tX, Y = (int)a.tY, Z = (int)a.tZ;
   const int sz[3] = {(int)a.sX, (int)a.sY, (int)a.sZ};
 
-  // ---- samples of this thread: coordinates, in-bound flag, taps ----
-  int px[TL_PER_THREAD], py[TL_PER_THREAD], pz[TL_PER_THREAD];
-  bool valid[TL_PER_THREAD], inb[TL_PER_THREAD];
-  constexpr bool GATHER = TileMeta<MODE>::gather, DERIV = TileMeta<MODE>::deriv;
-  constexpr int NG = DERIV ? NT : 1;
-  int lo[TL_PER_THREAD][3];
-  float w[TL_PER_THREAD][3][NT];
-  float gw[TL_PER_THREAD][3][NG];   // derivative weights (gradient modes)
+  // ---- samples of this thread: in-bound flags, taps ----
+  // sample i is (px, py + i * SDY, pz0 + i * SDZ)
+  int sx, sy, sz0;
+  tl_thread<B>(sx, sy, sz0);
+  const int px = tx * B::TX + sx, py = ty * B::TY + sy, pz0 = tz * B::TZ + sz0;
+  bool valid[R], inb[R];
+  // pull / push / count keep every sample's weights; gradient modes keep
+  // the coordinates and compute one sample's weights and derivatives at a
+  // time (registers)
+  constexpr int NW = DERIV ? 1 : R;
+  int lo[R][3];
+  float w[NW][3][NT];
+  float xq[R][3];
   int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
   const float tol = (a.extrap & 2) ? 0.55f : 0.05f;
+  const int gD = (int)a.grid_s[4];
+  const int gS = (int)(B::SDY * a.grid_s[2] + B::SDZ * a.grid_s[3]);
+  const float *gp0 = a.grid + (int)(n * a.grid_s[0] + px * a.grid_s[1] + py * a.grid_s[2] + pz0 * a.grid_s[3]);
 #pragma unroll
-  for (int k = 0; k < TL_PER_THREAD; ++k) {
-    const int s = threadIdx.x + k * NI_BLOCK;
-    px[k] = tx * TL_X + s / (TL_Y * TL_Z);
-    py[k] = ty * TL_Y + (s / TL_Z) % TL_Y;
-    pz[k] = tz * TL_Z + s % TL_Z;
-    valid[k] = px[k] < X && py[k] < Y && pz[k] < Z;
-    inb[k] = false;
-    if (valid[k]) {
-      const float *gp = a.grid + (int)(n * a.grid_s[0] + px[k] * a.grid_s[1] + py[k] * a.grid_s[2] + pz[k] * a.grid_s[3]);
-      const int gD = (int)a.grid_s[4];
+  for (int i = 0; i < R; ++i) {
+    valid[i] = px < X && py + i * B::SDY < Y && pz0 + i * B::SDZ < Z;
+    inb[i] = false;
+    if (valid[i]) {
+      const float *gp = gp0 + i * gS;
       const float xs[3] = {gp[0], gp[gD], gp[2 * gD]};
       bool ok = true;
       if (!(a.extrap & 1))
 #pragma unroll
         for (int d = 0; d < 3; ++d) ok = ok && (xs[d] >= -tol && xs[d] < (float)(sz[d] - 1) + tol);
-      inb[k] = ok;
+      inb[i] = ok;
       if (ok) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          float gtmp[NT];
-          lo[k][d] = tl_taps<ORD, DERIV>(xs[d], w[k][d], gtmp);
+          float wtmp[NT], gtmp[NT];
+          xq[i][d] = xs[d];
+          lo[i][d] = tl_taps<ORD>(xs[d], wtmp, gtmp);
+          if constexpr (!DERIV) {
 #pragma unroll
-          for (int t = 0; t < NG; ++t) gw[k][d][t] = gtmp[t];
-          mn[d] = min(mn[d], lo[k][d]);
-          mx[d] = max(mx[d], lo[k][d]);
+            for (int t = 0; t < NT; ++t) w[i][d][t] = wtmp[t];
+          }
+          mn[d] = min(mn[d], lo[i][d]);
+          mx[d] = max(mx[d], lo[i][d]);
         }
       }
     }
@@ -199,23 +315,22 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
   }
 
   const int oC = (int)a.out_s[1], oK = (int)a.out_s[5];
+  const int oS = (int)(B::SDY * a.out_s[3] + B::SDZ * a.out_s[4]);   // between a thread's samples
   if (mn[0] > mx[0]) {   // no in-bound sample in this tile
     if (MODE == TM_PULL || MODE == TM_SGRAD) {
       const int C = (int)a.C, nk = MODE == TM_SGRAD ? 3 : 1;
+      float *op0 = a.out + (int)(n * a.out_s[0] + px * a.out_s[2] + py * a.out_s[3] + pz0 * a.out_s[4]);
 #pragma unroll
-      for (int k = 0; k < TL_PER_THREAD; ++k)
-        if (valid[k]) {
-          float *op = a.out + (int)(n * a.out_s[0] + px[k] * a.out_s[2] + py[k] * a.out_s[3] + pz[k] * a.out_s[4]);
+      for (int i = 0; i < R; ++i)
+        if (valid[i])
           for (int c = 0; c < C; ++c)
-            for (int d = 0; d < nk; ++d) op[c * oC + d * oK] = 0.f;
-        }
+            for (int d = 0; d < nk; ++d) op0[i * oS + c * oC + d * oK] = 0.f;
     } else if (MODE == TM_GGRAD || MODE == TM_GGRAD1) {
+      float *gq = a.gout + (int)(n * a.gout_s[0] + px * a.gout_s[1] + py * a.gout_s[2] + pz0 * a.gout_s[3]);
 #pragma unroll
-      for (int k = 0; k < TL_PER_THREAD; ++k)
-        if (valid[k]) {
-          float *gp = a.gout + (int)(n * a.gout_s[0] + px[k] * a.gout_s[1] + py[k] * a.gout_s[2] + pz[k] * a.gout_s[3]);
-          for (int d = 0; d < 3; ++d) gp[d * a.gout_s[4]] = 0.f;
-        }
+      for (int i = 0; i < R; ++i)
+        if (valid[i])
+          for (int d = 0; d < 3; ++d) gq[i * (B::SDY * a.gout_s[2] + B::SDZ * a.gout_s[3]) + d * a.gout_s[4]] = 0.f;
     }
     return;
   }
@@ -228,17 +343,17 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
     ext[d] = (int)(e < 0x7fffffff ? e : 0x7fffffff);
   }
   if (!fits) return;   // scattered tile: left to tile_fallback
-  int off[TL_PER_THREAD];
+  int off[R];
 #pragma unroll
-  for (int k = 0; k < TL_PER_THREAD; ++k)
-    off[k] = inb[k] ? ((lo[k][0] - mn[0]) * DY + (lo[k][1] - mn[1])) * DZ + (lo[k][2] - mn[2]) : 0;
+  for (int i = 0; i < R; ++i)
+    off[i] = inb[i] ? (lo[i][0] - mn[0]) * DYZ + (lo[i][1] - mn[1]) * DZ + (lo[i][2] - mn[2]) : 0;
 
   const int C = MODE == TM_COUNT ? 1 : (int)a.C;
   // scatter: number of in-bound samples (bounds every fixed-point box sum)
   int n_in = 0;
   if (!GATHER) {
 #pragma unroll
-    for (int k = 0; k < TL_PER_THREAD; ++k) n_in += __syncthreads_count(inb[k]);
+    for (int i = 0; i < R; ++i) n_in += __syncthreads_count(inb[i]);
   }
   __shared__ float s_red[NI_BLOCK / 64][NCC];
   __shared__ float s_scale[NCC], s_inv[NCC];
@@ -265,10 +380,17 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
     }
   }
   __syncthreads();
+  TilePlane<B> pl;
+  pl.init(ext[1], ext[2], fofs, fsg);
+  if (!GATHER) {   // scatter box: cleared once; each flush clears what it read
+    int *ibox = (int *)box;
+    for (int i = threadIdx.x; i < NCC * VOL; i += NI_BLOCK) ibox[i] = 0;
+    __syncthreads();
+  }
 
-  float og[TL_PER_THREAD][3];   // grid gradient (TM_GGRAD*), summed over chunks
+  float og[R][3];   // grid gradient (TM_GGRAD*), summed over chunks
 #pragma unroll
-  for (int k = 0; k < TL_PER_THREAD; ++k) og[k][0] = og[k][1] = og[k][2] = 0.f;
+  for (int i = 0; i < R; ++i) og[i][0] = og[i][1] = og[i][2] = 0.f;
 
   for (int c0 = 0; c0 < C; c0 += NCC) {
     const int nc = (C - c0) < NCC ? (C - c0) : NCC;
@@ -276,70 +398,59 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
     // keep the per-tap weight products inside the chunk loop (hoisting
     // them would hold (ORD + 1)^3 products per sample in registers)
 #pragma unroll
-    for (int k = 0; k < TL_PER_THREAD; ++k)
+    for (int i = 0; i < NW; ++i)
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
 #pragma unroll
-        for (int t = 0; t < NT; ++t) asm volatile("" : "+v"(w[k][d][t]));
-#pragma unroll
-        for (int t = 0; t < NG; ++t) asm volatile("" : "+v"(gw[k][d][t]));
+        for (int t = 0; t < NT; ++t) asm volatile("" : "+v"(w[i][d][t]));
       }
-    const int nbox = nc * VOL;
 
     if (GATHER) {
       // ---- stage the folded, signed source box ----
+      // only the ext[0] x ext[1] x ext[2] corner of the box is ever read:
+      // each thread fills its y-z plane positions for every channel and x
       const float *src = a.src + (int)(n * a.src_s[0] + c0 * a.src_s[1]);
       const int sC = (int)a.src_s[1];
-      // only the ext[0] x ext[1] x ext[2] corner of the box is ever read:
-      // enumerate it directly (floor divisions by float reciprocals, exact
-      // after the +-1 fix-ups for indices below 2^22)
-      const int E2 = ext[2], E12 = ext[1] * ext[2], E = ext[0] * E12;
-      const float rE = 1.f / (float)E, r12 = 1.f / (float)E12, r2 = 1.f / (float)E2;
-      const int nfill = nc * E;
-#pragma unroll 4
-      for (int i = threadIdx.x; i < nfill; i += NI_BLOCK) {
-        int c = (int)((float)i * rE);
-        c -= c * E > i;
-        c += (c + 1) * E <= i;
-        const int rr = i - c * E;
-        int bx = (int)((float)rr * r12);
-        bx -= bx * E12 > rr;
-        bx += (bx + 1) * E12 <= rr;
-        const int rem = rr - bx * E12;
-        int by = (int)((float)rem * r2);
-        by -= by * E2 > rem;
-        by += (by + 1) * E2 <= rem;
-        const int bz = rem - by * E2;
-        const float sg = fsg[bx] * fsg[B::DX + by] * fsg[B::DX + DY + bz];
-        float v = 0.f;
-        if (sg != 0.f) v = src[c * sC + fofs[bx] + fofs[B::DX + by] + fofs[B::DX + DY + bz]] * sg;
-        box[c * VOL + (bx * DY + by) * DZ + bz] = v;
+#pragma unroll
+      for (int j = 0; j < B::NPP; ++j) {
+        if (pl.yz_lds[j] < 0) continue;
+#pragma unroll
+        for (int c = 0; c < NCC; ++c) {
+          if (c >= nc) break;
+#pragma unroll
+          for (int bx = 0; bx < B::DX; ++bx) {
+            if (bx >= ext[0]) break;
+            const float sg = fsg[bx] * pl.yz_sg[j];
+            float v = 0.f;
+            if (sg != 0.f) v = src[c * sC + fofs[bx] + pl.yz_ofs[j]] * sg;
+            box[c * VOL + bx * DYZ + pl.yz_lds[j]] = v;
+          }
+        }
       }
-    } else {
-      for (int i = threadIdx.x; i < NCC * VOL; i += NI_BLOCK) box[i] = 0.f;
     }
     // scatter: values of this chunk and the per-channel fixed-point scales
-    float tv[TL_PER_THREAD][NCC];
+    float tv[R][NCC];
     if (!GATHER) {
       float mloc[NCC];
 #pragma unroll
       for (int c = 0; c < NCC; ++c) mloc[c] = 0.f;
+      const float *tp0 = MODE == TM_PUSH ? a.trgt + (int)(n * a.trgt_s[0] + (int64_t)c0 * a.trgt_s[1] +
+                                                          px * a.trgt_s[2] + py * a.trgt_s[3] + pz0 * a.trgt_s[4])
+                                         : nullptr;
+      const int tC = (int)a.trgt_s[1], tS = (int)(B::SDY * a.trgt_s[3] + B::SDZ * a.trgt_s[4]);
 #pragma unroll
-      for (int k = 0; k < TL_PER_THREAD; ++k) {
+      for (int i = 0; i < R; ++i) {
 #pragma unroll
-        for (int c = 0; c < NCC; ++c) tv[k][c] = 0.f;
-        if (!inb[k]) continue;
+        for (int c = 0; c < NCC; ++c) tv[i][c] = 0.f;
+        if (!inb[i]) continue;
         if (MODE == TM_PUSH) {
-          const float *tp = a.trgt + (int)(n * a.trgt_s[0] + (int64_t)c0 * a.trgt_s[1] + px[k] * a.trgt_s[2] +
-                                           py[k] * a.trgt_s[3] + pz[k] * a.trgt_s[4]);
-          const int tC = (int)a.trgt_s[1];
 #pragma unroll
-          for (int c = 0; c < NCC; ++c) tv[k][c] = c < nc ? tp[c * tC] : 0.f;
+          for (int c = 0; c < NCC; ++c) tv[i][c] = c < nc ? tp0[i * tS + c * tC] : 0.f;
         } else {
-          tv[k][0] = 1.f;
+          tv[i][0] = 1.f;
         }
 #pragma unroll
-        for (int c = 0; c < NCC; ++c) mloc[c] = fmaxf(mloc[c], fabsf(tv[k][c]));
+        for (int c = 0; c < NCC; ++c) mloc[c] = fmaxf(mloc[c], fabsf(tv[i][c]));
       }
 #pragma unroll
       for (int c = 0; c < NCC; ++c) {
@@ -366,70 +477,103 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
     }
     __syncthreads();
 
+    if (MODE == TM_PULL) {
+      float acc[R][NCC];
 #pragma unroll
-    for (int k = 0; k < TL_PER_THREAD; ++k) {
-      if (MODE == TM_PULL) {
-        if (!valid[k]) continue;
-        float acc[NCC];
+      for (int i = 0; i < R; ++i)
 #pragma unroll
-        for (int c = 0; c < NCC; ++c) acc[c] = 0.f;
-        if (inb[k]) {
-          // z innermost: consecutive taps are adjacent LDS words (paired
-          // reads); the sum order differs from pp_sample's (z outer) only
-          // by rounding
-          const float *bp = box + off[k];
-          // kx rolled: keeps the LDS reads of one (kx, ky) row in flight
-          // without the unrolled nest's register demand
+        for (int c = 0; c < NCC; ++c) acc[i][c] = 0.f;
+      {
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          if (!inb[i]) continue;
+          const float *bp = box + off[i];
 #pragma unroll 1
-          for (int kx = 0; kx < NT; ++kx)
+          for (int kx = 0; kx < NT; ++kx, bp += DYZ) {
 #pragma unroll
             for (int ky = 0; ky < NT; ++ky) {
-              const float wxy = w[k][0][kx] * w[k][1][ky];
+              const float wxy = w[i][0][0] * w[i][1][ky];
 #pragma unroll
               for (int kz = 0; kz < NT; ++kz) {
-                const float ww = wxy * w[k][2][kz];
-                const int o = kx * DYZ + ky * DZ + kz;
+                const float ww = wxy * w[i][2][kz];
+                const int o = ky * DZ + kz;
 #pragma unroll
-                for (int c = 0; c < NCC; ++c) acc[c] += bp[c * VOL + o] * ww;
+                for (int c = 0; c < NCC; ++c) acc[i][c] += bp[c * VOL + o] * ww;
               }
-              // bound the LDS reads in flight to one row of taps
               __builtin_amdgcn_sched_barrier(0);
             }
+            tl_rot(w[i][0]);
+          }
         }
-        float *op = a.out + (int)(n * a.out_s[0] + (int64_t)c0 * oC + px[k] * a.out_s[2] + py[k] * a.out_s[3] +
-                                  pz[k] * a.out_s[4]);
+      }
+      float *op0 = a.out + (int)(n * a.out_s[0] + (int64_t)c0 * oC + px * a.out_s[2] + py * a.out_s[3] +
+                                 pz0 * a.out_s[4]);
 #pragma unroll
-        for (int c = 0; c < NCC; ++c)
-          if (c < nc) op[c * oC] = acc[c];
-      } else if (MODE == TM_SGRAD) {
-        if (!valid[k]) continue;
+      for (int i = 0; i < R; ++i)
+        if (valid[i])
+#pragma unroll
+          for (int c = 0; c < NCC; ++c)
+            if (c < nc) op0[i * oS + c * oC] = acc[i][c];
+    } else if (MODE == TM_SGRAD) {
+      // samples one at a time (rolled): element 0 of the per-sample state
+      // is the current sample, rotated after each (R rotations restore it)
+#pragma unroll 1
+      for (int i = 0; i < R; ++i) {
+        const bool vi = valid[0], ii = inb[0];
+        const int oi = off[0];
+        const float q0 = xq[0][0], q1 = xq[0][1], q2 = xq[0][2];
+        tl_rot(valid);
+        tl_rot(inb);
+        tl_rot(off);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const float t = xq[0][d];
+#pragma unroll
+          for (int k = 0; k + 1 < R; ++k) xq[k][d] = xq[k + 1][d];
+          xq[R - 1][d] = t;
+        }
+        if (!vi) continue;
         float ax[NCC], ay[NCC], az[NCC];
 #pragma unroll
         for (int c = 0; c < NCC; ++c) ax[c] = ay[c] = az[c] = 0.f;
-        if (inb[k]) {
-          const float *bp = box + off[k];
+        if (ii) {
+          float wi[3][NT], gi[3][NT];
+          tl_taps<ORD, true>(q0, wi[0], gi[0]);
+          tl_taps<ORD, true>(q1, wi[1], gi[1]);
+          tl_taps<ORD, true>(q2, wi[2], gi[2]);
+          const float *bp = box + oi;
 #pragma unroll 1
-          for (int kz = 0; kz < NT; ++kz)
+          // separable: per (z, y) tap row, the x sums with weights and
+          // derivatives, then the y-z factors (x inner, y, z rolled)
+          for (int kz = 0; kz < NT; ++kz, ++bp) {
+            const float *bq = bp;
+#pragma unroll 1
+            for (int ky = 0; ky < NT; ++ky, bq += DZ) {
+              tl_opaque(wi[0]);
+              tl_opaque(gi[0]);
+              const float wyz = wi[1][0] * wi[2][0], gyz = gi[1][0] * wi[2][0], wgz = wi[1][0] * gi[2][0];
 #pragma unroll
-            for (int ky = 0; ky < NT; ++ky)
+              for (int c = 0; c < NCC; ++c) {
+                float sw = 0.f, sd = 0.f;
 #pragma unroll
-              for (int kx = 0; kx < NT; ++kx) {
-                const float wx = w[k][0][kx], wy = w[k][1][ky], wz = w[k][2][kz];
-                const float gx = (gw[k][0][kx] * wy) * wz, gy = (wx * gw[k][1][ky]) * wz,
-                            gz = (wx * wy) * gw[k][2][kz];
-                const int o = kx * DYZ + ky * DZ + kz;
-#pragma unroll
-                for (int c = 0; c < NCC; ++c) {
-                  const float v = bp[c * VOL + o];
-                  ax[c] += v * gx;
-                  ay[c] += v * gy;
-                  az[c] += v * gz;
+                for (int kx = 0; kx < NT; ++kx) {
+                  const float v = bq[c * VOL + kx * DYZ];
+                  sw += wi[0][kx] * v;
+                  sd += gi[0][kx] * v;
                 }
-                if (kx == NT - 1) __builtin_amdgcn_sched_barrier(0);
+                ax[c] += sd * wyz;
+                ay[c] += sw * gyz;
+                az[c] += sw * wgz;
               }
+              tl_rot(wi[1]);
+              tl_rot(gi[1]);
+            }
+            tl_rot(wi[2]);
+            tl_rot(gi[2]);
+          }
         }
-        float *op = a.out + (int)(n * a.out_s[0] + (int64_t)c0 * oC + px[k] * a.out_s[2] + py[k] * a.out_s[3] +
-                                  pz[k] * a.out_s[4]);
+        float *op = a.out + (int)(n * a.out_s[0] + (int64_t)c0 * oC + px * a.out_s[2] + py * a.out_s[3] +
+                                  pz0 * a.out_s[4]) + i * oS;
 #pragma unroll
         for (int c = 0; c < NCC; ++c)
           if (c < nc) {
@@ -437,12 +581,33 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
             op[c * oC + oK] = ay[c];
             op[c * oC + 2 * oK] = az[c];
           }
-      } else if (MODE == TM_GGRAD || MODE == TM_GGRAD1) {
-        if (!inb[k]) continue;
+      }
+    } else if (MODE == TM_GGRAD || MODE == TM_GGRAD1) {
+      // samples one at a time (rolled), as in TM_SGRAD; og rotates with them
+#pragma unroll 1
+      for (int i = 0; i < R; ++i) {
+        const bool ii = inb[0];
+        const int oi = off[0];
+        const float q0 = xq[0][0], q1 = xq[0][1], q2 = xq[0][2];
+        float o0 = og[0][0], o1 = og[0][1], o2 = og[0][2];
+        tl_rot(inb);
+        tl_rot(off);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const float t = xq[0][d];
+#pragma unroll
+          for (int k = 0; k + 1 < R; ++k) {
+            xq[k][d] = xq[k + 1][d];
+            og[k][d] = og[k + 1][d];
+          }
+          xq[R - 1][d] = t;
+        }
+        if (ii) {
         float tvv[NCC];
         if (MODE == TM_GGRAD) {
-          const float *tp = a.trgt + (int)(n * a.trgt_s[0] + (int64_t)c0 * a.trgt_s[1] + px[k] * a.trgt_s[2] +
-                                           py[k] * a.trgt_s[3] + pz[k] * a.trgt_s[4]);
+          const float *tp = a.trgt + (int)(n * a.trgt_s[0] + (int64_t)c0 * a.trgt_s[1] + px * a.trgt_s[2] +
+                                           py * a.trgt_s[3] + pz0 * a.trgt_s[4]) +
+                            i * (int)(B::SDY * a.trgt_s[3] + B::SDZ * a.trgt_s[4]);
           const int tC = (int)a.trgt_s[1];
 #pragma unroll
           for (int c = 0; c < NCC; ++c) tvv[c] = c < nc ? tp[c * tC] : 0.f;
@@ -450,84 +615,108 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
 #pragma unroll
           for (int c = 0; c < NCC; ++c) tvv[c] = c < nc ? 1.f : 0.f;
         }
-        const float *bp = box + off[k];
+        float wi[3][NT], gi[3][NT];
+        tl_taps<ORD, true>(q0, wi[0], gi[0]);
+        tl_taps<ORD, true>(q1, wi[1], gi[1]);
+        tl_taps<ORD, true>(q2, wi[2], gi[2]);
+        const float *bp = box + oi;
 #pragma unroll 1
-        for (int kz = 0; kz < NT; ++kz)
-#pragma unroll
-          for (int ky = 0; ky < NT; ++ky)
+        // separable, as in TM_SGRAD, on the channel dot products
+        for (int kz = 0; kz < NT; ++kz, ++bp) {
+          const float *bq = bp;
+#pragma unroll 1
+          for (int ky = 0; ky < NT; ++ky, bq += DZ) {
+            tl_opaque(wi[0]);
+            tl_opaque(gi[0]);
+            const float wyz = wi[1][0] * wi[2][0], gyz = gi[1][0] * wi[2][0], wgz = wi[1][0] * gi[2][0];
+            float sw = 0.f, sd = 0.f;
 #pragma unroll
             for (int kx = 0; kx < NT; ++kx) {
-              const float wx = w[k][0][kx], wy = w[k][1][ky], wz = w[k][2][kz];
-              const float gx = (gw[k][0][kx] * wy) * wz, gy = (wx * gw[k][1][ky]) * wz,
-                          gz = (wx * wy) * gw[k][2][kz];
-              const int o = kx * DYZ + ky * DZ + kz;
               float dot = 0.f;
               // channels >= nc of a partial chunk hold stale LDS: select, not multiply by 0
 #pragma unroll
-              for (int c = 0; c < NCC; ++c) dot += c < nc ? bp[c * VOL + o] * tvv[c] : 0.f;
-              og[k][0] += gx * dot;
-              og[k][1] += gy * dot;
-              og[k][2] += gz * dot;
-              if (kx == NT - 1) __builtin_amdgcn_sched_barrier(0);
+              for (int c = 0; c < NCC; ++c) dot += c < nc ? bq[c * VOL + kx * DYZ] * tvv[c] : 0.f;
+              sw += wi[0][kx] * dot;
+              sd += gi[0][kx] * dot;
             }
-      } else {
-        if (!inb[k]) continue;
-        float sv[NCC];
-#pragma unroll
-        for (int c = 0; c < NCC; ++c) sv[c] = tv[k][c] * s_scale[c];   // exact (power of two)
-        int *bp = (int *)box + off[k];
-#pragma unroll
-        for (int kz = 0; kz < NT; ++kz)
-#pragma unroll
-          for (int ky = 0; ky < NT; ++ky)
-#pragma unroll
-            for (int kx = 0; kx < NT; ++kx) {
-              const float ww = (w[k][0][kx] * w[k][1][ky]) * w[k][2][kz];
-              const int o = kx * DYZ + ky * DZ + kz;
-#pragma unroll
-              for (int c = 0; c < NCC; ++c) atomicAdd(bp + c * VOL + o, (int)rintf(ww * sv[c]));
-            }
+            o0 += sd * wyz;
+            o1 += sw * gyz;
+            o2 += sw * wgz;
+            tl_rot(wi[1]);
+            tl_rot(gi[1]);
+          }
+          tl_rot(wi[2]);
+          tl_rot(gi[2]);
+        }
+        }
+        og[R - 1][0] = o0;
+        og[R - 1][1] = o1;
+        og[R - 1][2] = o2;
       }
-    }
+    } else {
+      // push / count: fixed-point splats (exact: the scale is a power of two)
+      float sv[R][NCC];
+#pragma unroll
+      for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int c = 0; c < NCC; ++c) sv[i][c] = tv[i][c] * s_scale[c];
+      {
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          if (!inb[i]) continue;
+          int *bp = (int *)box + off[i];
+#pragma unroll 1
+          for (int kx = 0; kx < NT; ++kx, bp += DYZ) {
+#pragma unroll
+            for (int ky = 0; ky < NT; ++ky) {
+              const float wxy = w[i][0][0] * w[i][1][ky];
+#pragma unroll
+              for (int kz = 0; kz < NT; ++kz) {
+                const float ww = wxy * w[i][2][kz];
+                const int o = ky * DZ + kz;
+#pragma unroll
+                for (int c = 0; c < NCC; ++c) atomicAdd(bp + c * VOL + o, (int)rintf(ww * sv[i][c]));
+              }
+            }
+            tl_rot(w[i][0]);
+          }
+        }
+      }
 
-    if (!GATHER) {
-      // ---- add the box to the output, folded and signed ----
+      // ---- add the box to the output, folded and signed; clear it for
+      // the next chunk (the splats never leave the ext corner) ----
       __syncthreads();
       float *out = a.out + (int)(n * a.out_s[0] + (int64_t)c0 * oC);
-      const int *ibox = (const int *)box;
-      const int E2 = ext[2], E12 = ext[1] * ext[2], E = ext[0] * E12;
-      const float rE = 1.f / (float)E, r12 = 1.f / (float)E12, r2 = 1.f / (float)E2;
-      const int nflush = nc * E;
-      for (int i = threadIdx.x; i < nflush; i += NI_BLOCK) {
-        int c = (int)((float)i * rE);
-        c -= c * E > i;
-        c += (c + 1) * E <= i;
-        const int rr = i - c * E;
-        int bx = (int)((float)rr * r12);
-        bx -= bx * E12 > rr;
-        bx += (bx + 1) * E12 <= rr;
-        const int rem = rr - bx * E12;
-        int by = (int)((float)rem * r2);
-        by -= by * E2 > rem;
-        by += (by + 1) * E2 <= rem;
-        const int bz = rem - by * E2;
-        const int iv = ibox[c * VOL + (bx * DY + by) * DZ + bz];
-        if (iv == 0) continue;
-        const float v = (float)iv * s_inv[c];
-        const float sg = fsg[bx] * fsg[B::DX + by] * fsg[B::DX + DY + bz];
-        if (sg != 0.f) atomic_add(out + c * oC + fofs[bx] + fofs[B::DX + by] + fofs[B::DX + DY + bz], v * sg);
+      int *ibox = (int *)box;
+#pragma unroll
+      for (int j = 0; j < B::NPP; ++j) {
+        if (pl.yz_lds[j] < 0) continue;
+#pragma unroll
+        for (int c = 0; c < NCC; ++c) {
+          if (c >= nc) break;
+#pragma unroll
+          for (int bx = 0; bx < B::DX; ++bx) {
+            if (bx >= ext[0]) break;
+            int *bq = ibox + c * VOL + bx * DYZ + pl.yz_lds[j];
+            const int iv = *bq;
+            if (iv == 0) continue;
+            *bq = 0;
+            const float sg = fsg[bx] * pl.yz_sg[j];
+            if (sg != 0.f) atomic_add(out + c * oC + fofs[bx] + pl.yz_ofs[j], ((float)iv * s_inv[c]) * sg);
+          }
+        }
       }
     }
   }
   if (MODE == TM_GGRAD || MODE == TM_GGRAD1) {
+    float *gq = a.gout + (int)(n * a.gout_s[0] + px * a.gout_s[1] + py * a.gout_s[2] + pz0 * a.gout_s[3]);
+    const int gZ = (int)(B::SDY * a.gout_s[2] + B::SDZ * a.gout_s[3]), gDo = (int)a.gout_s[4];
 #pragma unroll
-    for (int k = 0; k < TL_PER_THREAD; ++k)
-      if (valid[k]) {
-        float *gp = a.gout + (int)(n * a.gout_s[0] + px[k] * a.gout_s[1] + py[k] * a.gout_s[2] + pz[k] * a.gout_s[3]);
-        const int gD = (int)a.gout_s[4];
-        gp[0] = og[k][0];
-        gp[gD] = og[k][1];
-        gp[2 * gD] = og[k][2];
+    for (int i = 0; i < R; ++i)
+      if (valid[i]) {
+        gq[i * gZ] = og[i][0];
+        gq[i * gZ + gDo] = og[i][1];
+        gq[i * gZ + 2 * gDo] = og[i][2];
       }
   }
 }
@@ -541,7 +730,8 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_fallback(PPArgs<float> a, int n
                                                           int64_t nblocks) {
   constexpr int NT = ORD + 1;
   constexpr int FL = TileFlags<MODE>::value;
-  using B = TileBox<ORD>;
+  using B = TileBox<ORD, !TileMeta<MODE>::gather>;
+  constexpr int R = B::R;
   __shared__ int red[NI_BLOCK / 64][6];
   int64_t r = xcd_block(nblocks);
   const int tz = (int)(r % ntz); r /= ntz;
@@ -550,20 +740,20 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_fallback(PPArgs<float> a, int n
   const int n = (int)(r / ntx);
   const int X = (int)a.tX, Y = (int)a.tY, Z = (int)a.tZ;
   const int sz[3] = {(int)a.sX, (int)a.sY, (int)a.sZ};
-  int px[TL_PER_THREAD], py[TL_PER_THREAD], pz[TL_PER_THREAD];
-  bool valid[TL_PER_THREAD];
+  int sx, sy, sz0;
+  tl_thread<B>(sx, sy, sz0);
+  const int px = tx * B::TX + sx, py = ty * B::TY + sy, pz0 = tz * B::TZ + sz0;
+  bool valid[R];
   int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
   const float tol = (a.extrap & 2) ? 0.55f : 0.05f;
+  const int gD = (int)a.grid_s[4];
+  const int gS = (int)(B::SDY * a.grid_s[2] + B::SDZ * a.grid_s[3]);
+  const float *gp0 = a.grid + (int)(n * a.grid_s[0] + px * a.grid_s[1] + py * a.grid_s[2] + pz0 * a.grid_s[3]);
 #pragma unroll
-  for (int k = 0; k < TL_PER_THREAD; ++k) {
-    const int s = threadIdx.x + k * NI_BLOCK;
-    px[k] = tx * TL_X + s / (TL_Y * TL_Z);
-    py[k] = ty * TL_Y + (s / TL_Z) % TL_Y;
-    pz[k] = tz * TL_Z + s % TL_Z;
-    valid[k] = px[k] < X && py[k] < Y && pz[k] < Z;
-    if (valid[k]) {
-      const float *gp = a.grid + (int)(n * a.grid_s[0] + px[k] * a.grid_s[1] + py[k] * a.grid_s[2] + pz[k] * a.grid_s[3]);
-      const int gD = (int)a.grid_s[4];
+  for (int i = 0; i < R; ++i) {
+    valid[i] = px < X && py + i * B::SDY < Y && pz0 + i * B::SDZ < Z;
+    if (valid[i]) {
+      const float *gp = gp0 + i * gS;
       const float xs[3] = {gp[0], gp[gD], gp[2 * gD]};
       bool ok = true;
       if (!(a.extrap & 1))
@@ -607,28 +797,26 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_fallback(PPArgs<float> a, int n
     fits = fits && e <= (d == 0 ? B::DX : d == 1 ? B::DY : B::DZ);
   }
   if (fits) return;
-#pragma unroll
-  for (int k = 0; k < TL_PER_THREAD; ++k)
-    if (valid[k]) pp_sample<float, int32_t, ORD, FL, true>(a, n, px[k], py[k], pz[k]);
+#pragma unroll 1
+  for (int i = 0; i < R; ++i)
+    if (valid[i]) pp_sample<float, int32_t, ORD, FL, true>(a, n, px, py + i * B::SDY, pz0 + i * B::SDZ);
 }
 
 template <int ORD, int MODE>
 static void launch_tile(const PPArgs<float> &a, hipStream_t st) {
-  const int ntx = (int)((a.tX + TL_X - 1) / TL_X);
-  const int nty = (int)((a.tY + TL_Y - 1) / TL_Y);
-  const int ntz = (int)((a.tZ + TL_Z - 1) / TL_Z);
+  using B = TileBox<ORD, !TileMeta<MODE>::gather>;
+  const int ntx = (int)((a.tX + B::TX - 1) / B::TX);
+  const int nty = (int)((a.tY + B::TY - 1) / B::TY);
+  const int ntz = (int)((a.tZ + B::TZ - 1) / B::TZ);
   const int64_t nb = a.N * (int64_t)ntx * nty * ntz;
   const dim3 g((unsigned)nb), b(NI_BLOCK);
-  constexpr int M = TileBox<ORD>::MAX_NCC;
   if constexpr (MODE == TM_COUNT) {
     hipLaunchKernelGGL((tile_kernel<ORD, MODE, 1>), g, b, 0, st, a, ntx, nty, ntz, nb);
   } else {
     if (a.C == 1)
       hipLaunchKernelGGL((tile_kernel<ORD, MODE, 1>), g, b, 0, st, a, ntx, nty, ntz, nb);
-    else if (a.C == 2 || M == 2)
-      hipLaunchKernelGGL((tile_kernel<ORD, MODE, (M < 2 ? M : 2)>), g, b, 0, st, a, ntx, nty, ntz, nb);
     else
-      hipLaunchKernelGGL((tile_kernel<ORD, MODE, M>), g, b, 0, st, a, ntx, nty, ntz, nb);
+      hipLaunchKernelGGL((tile_kernel<ORD, MODE, B::MAX_NCC>), g, b, 0, st, a, ntx, nty, ntz, nb);
   }
   hipLaunchKernelGGL((tile_fallback<ORD, MODE>), g, b, 0, st, a, ntx, nty, ntz, nb);
 }
