@@ -224,7 +224,7 @@ struct TileFlags {
 // is branch-free; lanes of a partial last chunk compute unused channels).
 template <int ORD, int MODE, int NCC>
 __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx, int nty, int ntz,
-                                                        int64_t nblocks) {
+                                                        int64_t nblocks, int *fb) {
   constexpr int NT = ORD + 1;
   constexpr bool GATHER = TileMeta<MODE>::gather, DERIV = TileMeta<MODE>::deriv;
   using B = TileBox<ORD, !GATHER>;
@@ -234,7 +234,8 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
   __shared__ float box[NCC * VOL];
   __shared__ int red[NI_BLOCK / 64][6];
 
-  int64_t r = xcd_block(nblocks);
+  const int64_t tile = xcd_block(nblocks);
+  int64_t r = tile;
   const int tz = (int)(r % ntz); r /= ntz;
   const int ty = (int)(r % nty); r /= nty;
   const int tx = (int)(r % ntx);
@@ -342,7 +343,10 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
     fits = fits && e <= (d == 0 ? B::DX : d == 1 ? DY : DZ);
     ext[d] = (int)(e < 0x7fffffff ? e : 0x7fffffff);
   }
-  if (!fits) return;   // scattered tile: left to tile_fallback
+  if (!fits) {   // scattered tile: listed for tile_fallback
+    if (fb && threadIdx.x == 0) fb[1 + atomicAdd(fb, 1)] = (int)tile;
+    return;
+  }
   int off[R];
 #pragma unroll
   for (int i = 0; i < R; ++i)
@@ -721,27 +725,45 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
   }
 }
 
-// Tiles whose box does not fit: the same tiling and bounding-box test as
-// tile_kernel (bit-identical, so every tile is handled by exactly one of the
-// two kernels), then the direct per-sample routine.  Kept in its own kernel
-// so that its register demand does not lower the occupancy of tile_kernel.
+// Tiles whose box does not fit, by the direct per-sample routine.  With a
+// list (fb: count, then the tile ids that tile_kernel found too scattered)
+// a fixed grid walks the list; without one, every tile repeats tile_kernel's
+// bounding-box test (bit-identical, so every tile is handled by exactly one
+// of the two kernels).  Kept apart from tile_kernel so that its register
+// demand does not lower that kernel's occupancy.
 template <int ORD, int MODE>
 __global__ __launch_bounds__(NI_BLOCK) void tile_fallback(PPArgs<float> a, int ntx, int nty, int ntz,
-                                                          int64_t nblocks) {
+                                                          int64_t nblocks, const int *fb) {
   constexpr int NT = ORD + 1;
   constexpr int FL = TileFlags<MODE>::value;
   using B = TileBox<ORD, !TileMeta<MODE>::gather>;
   constexpr int R = B::R;
+  const int X = (int)a.tX, Y = (int)a.tY, Z = (int)a.tZ;
+  int sx, sy, sz0;
+  tl_thread<B>(sx, sy, sz0);
+  if (fb) {
+    const int cnt = fb[0];
+    for (int64_t j = blockIdx.x; j < cnt; j += gridDim.x) {
+      int64_t r = fb[1 + j];
+      const int tz = (int)(r % ntz); r /= ntz;
+      const int ty = (int)(r % nty); r /= nty;
+      const int tx = (int)(r % ntx);
+      const int n = (int)(r / ntx);
+      const int px = tx * B::TX + sx, py = ty * B::TY + sy, pz0 = tz * B::TZ + sz0;
+#pragma unroll 1
+      for (int i = 0; i < R; ++i)
+        if (px < X && py + i * B::SDY < Y && pz0 + i * B::SDZ < Z)
+          pp_sample<float, int32_t, ORD, FL, true>(a, n, px, py + i * B::SDY, pz0 + i * B::SDZ);
+    }
+    return;
+  }
   __shared__ int red[NI_BLOCK / 64][6];
   int64_t r = xcd_block(nblocks);
   const int tz = (int)(r % ntz); r /= ntz;
   const int ty = (int)(r % nty); r /= nty;
   const int tx = (int)(r % ntx);
   const int n = (int)(r / ntx);
-  const int X = (int)a.tX, Y = (int)a.tY, Z = (int)a.tZ;
   const int sz[3] = {(int)a.sX, (int)a.sY, (int)a.sZ};
-  int sx, sy, sz0;
-  tl_thread<B>(sx, sy, sz0);
   const int px = tx * B::TX + sx, py = ty * B::TY + sy, pz0 = tz * B::TZ + sz0;
   bool valid[R];
   int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
@@ -802,6 +824,9 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_fallback(PPArgs<float> a, int n
     if (valid[i]) pp_sample<float, int32_t, ORD, FL, true>(a, n, px, py + i * B::SDY, pz0 + i * B::SDZ);
 }
 
+// tile_kernel, then tile_fallback on the tiles it listed as too scattered
+// (list in stream-ordered scratch; without it, tile_fallback repeats the
+// bounding-box test on every tile).
 template <int ORD, int MODE>
 static void launch_tile(const PPArgs<float> &a, hipStream_t st) {
   using B = TileBox<ORD, !TileMeta<MODE>::gather>;
@@ -810,15 +835,28 @@ static void launch_tile(const PPArgs<float> &a, hipStream_t st) {
   const int ntz = (int)((a.tZ + B::TZ - 1) / B::TZ);
   const int64_t nb = a.N * (int64_t)ntx * nty * ntz;
   const dim3 g((unsigned)nb), b(NI_BLOCK);
+  int *fb = nullptr;
+  if (hipMallocAsync((void **)&fb, sizeof(int) * (size_t)(nb + 1), st) != hipSuccess) {
+    (void)hipGetLastError();
+    fb = nullptr;
+  } else {
+    hipLaunchKernelGGL(fill_zero<int>, dim3(1), b, 0, st, fb, (int64_t)1);
+  }
   if constexpr (MODE == TM_COUNT) {
-    hipLaunchKernelGGL((tile_kernel<ORD, MODE, 1>), g, b, 0, st, a, ntx, nty, ntz, nb);
+    hipLaunchKernelGGL((tile_kernel<ORD, MODE, 1>), g, b, 0, st, a, ntx, nty, ntz, nb, fb);
   } else {
     if (a.C == 1)
-      hipLaunchKernelGGL((tile_kernel<ORD, MODE, 1>), g, b, 0, st, a, ntx, nty, ntz, nb);
+      hipLaunchKernelGGL((tile_kernel<ORD, MODE, 1>), g, b, 0, st, a, ntx, nty, ntz, nb, fb);
     else
-      hipLaunchKernelGGL((tile_kernel<ORD, MODE, B::MAX_NCC>), g, b, 0, st, a, ntx, nty, ntz, nb);
+      hipLaunchKernelGGL((tile_kernel<ORD, MODE, B::MAX_NCC>), g, b, 0, st, a, ntx, nty, ntz, nb, fb);
   }
-  hipLaunchKernelGGL((tile_fallback<ORD, MODE>), g, b, 0, st, a, ntx, nty, ntz, nb);
+  if (fb) {
+    const unsigned gf = (unsigned)(nb < 2048 ? nb : 2048);
+    hipLaunchKernelGGL((tile_fallback<ORD, MODE>), dim3(gf), b, 0, st, a, ntx, nty, ntz, nb, (const int *)fb);
+    (void)hipFreeAsync(fb, st);
+  } else {
+    hipLaunchKernelGGL((tile_fallback<ORD, MODE>), g, b, 0, st, a, ntx, nty, ntz, nb, (const int *)nullptr);
+  }
 }
 
 }  // namespace ni_amd
