@@ -413,21 +413,26 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
       // ---- stage the folded, signed source box ----
       // only the ext[0] x ext[1] x ext[2] corner of the box is ever read:
       // each thread fills its y-z plane positions for every channel and x
+      // one channel row at a time: all DX loads issued before any store
+      // (positions past ext[0] re-read the last one: every folded offset is
+      // in range, so no load needs a guard)
       const float *src = a.src + (int)(n * a.src_s[0] + c0 * a.src_s[1]);
       const int sC = (int)a.src_s[1];
+      const int e0 = ext[0] - 1;
 #pragma unroll
       for (int j = 0; j < B::NPP; ++j) {
         if (pl.yz_lds[j] < 0) continue;
+#pragma unroll 1
+        for (int c = 0; c < nc; ++c) {
+          const float *sp = src + c * sC + pl.yz_ofs[j];
+          float v[B::DX];
 #pragma unroll
-        for (int c = 0; c < NCC; ++c) {
-          if (c >= nc) break;
+          for (int bx = 0; bx < B::DX; ++bx) v[bx] = sp[fofs[bx < e0 ? bx : e0]];
+          float *bq = box + c * VOL + pl.yz_lds[j];
 #pragma unroll
           for (int bx = 0; bx < B::DX; ++bx) {
-            if (bx >= ext[0]) break;
             const float sg = fsg[bx] * pl.yz_sg[j];
-            float v = 0.f;
-            if (sg != 0.f) v = src[c * sC + fofs[bx] + pl.yz_ofs[j]] * sg;
-            box[c * VOL + bx * DYZ + pl.yz_lds[j]] = v;
+            if (bx <= e0) bq[bx * DYZ] = sg != 0.f ? v[bx] * sg : 0.f;
           }
         }
       }
