@@ -697,21 +697,27 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
       __syncthreads();
       float *out = a.out + (int)(n * a.out_s[0] + (int64_t)c0 * oC);
       int *ibox = (int *)box;
+      const int e0 = ext[0] - 1;
 #pragma unroll
       for (int j = 0; j < B::NPP; ++j) {
         if (pl.yz_lds[j] < 0) continue;
+#pragma unroll 1
+        for (int c = 0; c < nc; ++c) {
+          // a whole x row of the box read (and cleared) at once, then its
+          // atomics issued back to back
+          int *bq = ibox + c * VOL + pl.yz_lds[j];
+          int iv[B::DX];
 #pragma unroll
-        for (int c = 0; c < NCC; ++c) {
-          if (c >= nc) break;
+          for (int bx = 0; bx < B::DX; ++bx) iv[bx] = bq[bx * DYZ];
+#pragma unroll
+          for (int bx = 0; bx < B::DX; ++bx) bq[bx * DYZ] = 0;
+          float *op = out + c * oC + pl.yz_ofs[j];
+          const float inv = s_inv[c];
 #pragma unroll
           for (int bx = 0; bx < B::DX; ++bx) {
-            if (bx >= ext[0]) break;
-            int *bq = ibox + c * VOL + bx * DYZ + pl.yz_lds[j];
-            const int iv = *bq;
-            if (iv == 0) continue;
-            *bq = 0;
             const float sg = fsg[bx] * pl.yz_sg[j];
-            if (sg != 0.f) atomic_add(out + c * oC + fofs[bx] + pl.yz_ofs[j], ((float)iv * s_inv[c]) * sg);
+            if (bx <= e0 && iv[bx] != 0 && sg != 0.f) atomic_add(op + fofs[bx], ((float)iv[bx] * inv) * sg);
+            if ((bx & 3) == 3) __builtin_amdgcn_sched_barrier(0);   // bound the addresses held at once
           }
         }
       }
