@@ -48,10 +48,10 @@ namespace ni_amd {
 //    4 x 8 x 16 (R = 2) above, where the weights need twice the registers.
 //  * The box covers the tile plus the ORD + 1 taps plus TL_SLACK voxels per
 //    axis.  Its x stride DYZP is 48 (mod 64) words and, for scatters, DZ is
-//    1 (mod R): the lanes of a wave reading or adding one tap then fall on
-//    distinct LDS banks (gathers: the four x-rows in the four quarters of
-//    the 64 banks; scatters: the R y-rows of a wave in the R residue
-//    classes mod R, its x-rows in disjoint slots of each class).
+//    1 (mod R): the 32 lanes of a half-wave reading or adding one tap then
+//    fall on the 32 distinct LDS banks (gathers: two x-rows of 16 z-samples,
+//    16 banks apart; scatters: R y-rows in the R residue classes mod R, two
+//    x-rows 16 banks apart in each class, R lanes R banks apart per row).
 template <int ORD, bool SCAT>
 struct TileBox {
   static constexpr int R = SCAT && ORD <= 3 ? 4 : 2;   // samples per thread
@@ -79,13 +79,19 @@ struct TileBox {
 };
 
 // First sample of this thread within its tile (sample i adds i * (SDY, SDZ)).
+// Scatters: y-row fastest within a wave, then x-row.
 template <class B>
 __device__ __forceinline__ void tl_thread(int &sx, int &sy, int &sz0) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int row = lane / B::ZL;
   sz0 = (lane % B::ZL) * (B::SDZ ? B::R : 1);
-  sx = row % B::RX + B::RX * (w % B::WX);
-  sy = B::SDZ ? row / B::RX + B::RY * (w / B::WX) : w;
+  if (B::SDZ) {
+    sx = row / B::RY + B::RX * (w % B::WX);
+    sy = row % B::RY + B::RY * (w / B::WX);
+  } else {
+    sx = row % B::RX + B::RX * (w % B::WX);
+    sy = w;
+  }
 }
 
 // The y-z plane of a box's ext[1] x ext[2] corner, split over the threads:
