@@ -39,31 +39,38 @@ namespace ni_amd {
 #define TL_SLACK 2     // extra box extent per axis for the spread of a tile's coordinates
 
 // Tile and box geometry of order ORD, fixed at compile time.
-//  * Gathers: 4 x 8 x 16 tiles; a wave holds 4 x-rows of 16 z-samples, and
-//    the two samples of a thread are 4 y-rows apart.
-//  * Scatters: larger tiles -- every box element is one global atomic in
-//    the flush, and a box exceeds its tile by ORD + 1 + TL_SLACK voxels per
-//    axis, so the atomics per sample fall with the tile size: 8 x 8 x 16
-//    (R = 4 samples per thread, consecutive along z) up to order 3, and
-//    4 x 8 x 16 (R = 2) above, where the weights need twice the registers.
-//  * The box covers the tile plus the ORD + 1 taps plus TL_SLACK voxels per
-//    axis.  Its x stride DYZP is 48 (mod 64) words and, for scatters, DZ is
-//    1 (mod R): the 32 lanes of a half-wave reading or adding one tap then
-//    fall on the 32 distinct LDS banks (gathers: two x-rows of 16 z-samples,
-//    16 banks apart; scatters: R y-rows in the R residue classes mod R, two
-//    x-rows 16 banks apart in each class, R lanes R banks apart per row).
+//  * Gathers: 256 threads, 4 x 8 x 16 tiles, two samples per thread WY
+//    y-rows apart (larger tiles re-read less halo but cost occupancy, which
+//    the gather needs more).  Scatters up to order 3: 512 threads,
+//    8 x 16 x 16 tiles, four z-consecutive samples per thread -- every box
+//    element is one global atomic in the flush and a box exceeds its tile
+//    by the ORD + 1 taps plus TL_SLACK voxels per axis, so the atomics per
+//    sample fall with the tile size.  Scatters above order 3 (twice the
+//    weight registers): 256 threads, 4 x 8 x 16, two samples per thread.
+//  * Lanes: gathers put 16 z-samples x 4 x-rows in a wave; scatters
+//    (ZL = TZ / R threads per z-row) R y-rows fastest, then x-rows.
+//  * The box is [c][x][y][z] with x stride DYZP padded to 48 (mod 64) words
+//    and, for scatters, DZ = 1 (mod R): the 32 lanes of a half-wave reading
+//    or adding one tap fall on the 32 distinct LDS banks (gathers: two
+//    x-rows of 16 z-samples, 16 banks apart; scatters: R y-rows in the R
+//    residue classes mod R, two x-rows 16 banks apart in each class, R
+//    lanes R banks apart per row).
 template <int ORD, bool SCAT>
 struct TileBox {
-  static constexpr int R = SCAT && ORD <= 3 ? 4 : 2;   // samples per thread
-  static constexpr int TX = SCAT && ORD <= 3 ? 8 : 4;
-  static constexpr int TY = 8;
+  static constexpr bool BIG = SCAT && ORD <= 3;
+  static constexpr int NTH = BIG ? 512 : 256;          // threads per work-group
+  static constexpr int NW = NTH / 64;                  // waves
+  static constexpr int R = BIG ? 4 : 2;                // samples per thread
+  static constexpr int TX = BIG ? 8 : 4;
+  static constexpr int TY = BIG ? 16 : 8;
   static constexpr int TZ = 16;
-  static constexpr int SDY = SCAT ? 0 : NI_BLOCK / 64;  // step between a thread's samples
+  static constexpr int ZL = SCAT ? TZ / R : TZ;        // threads per z-row of the tile
+  static constexpr int RY = SCAT ? R : 1;              // y-rows per wave
+  static constexpr int RX = 64 / ZL / RY;              // x-rows per wave
+  static constexpr int WX = TX / RX;                   // waves along x
+  static constexpr int WY = NW / WX;                   // waves along y
+  static constexpr int SDY = SCAT ? 0 : WY;            // step between a thread's samples
   static constexpr int SDZ = SCAT ? 1 : 0;
-  static constexpr int ZL = SCAT ? TZ / R : TZ;         // threads per z-row of the tile
-  static constexpr int RY = SCAT ? R : 1;               // y-rows per wave
-  static constexpr int RX = 64 / ZL / RY;               // x-rows per wave
-  static constexpr int WX = TX / RX;                    // waves along x
   static constexpr int DX = TX + ORD + TL_SLACK;
   static constexpr int DY = TY + ORD + TL_SLACK;
   static constexpr int DZ0 = TZ + ORD + TL_SLACK;
@@ -71,15 +78,17 @@ struct TileBox {
   static constexpr int DYZ = DY * DZ;
   static constexpr int DYZP = DYZ + ((48 - DYZ) % 64 + 64) % 64;
   static constexpr int VOL = DX * DYZP;
-  static constexpr int NPP = (DYZ + NI_BLOCK - 1) / NI_BLOCK;   // y-z plane positions per thread
-  static constexpr int MAX_NCC = 2;                     // channels staged at once
-  static_assert(TX * TY * TZ == NI_BLOCK * R, "R samples per thread");
-  static_assert(WX * RX == TX, "waves tile the x rows");
-  static_assert(SCAT ? (NI_BLOCK / 64 / WX) * RY == TY : (NI_BLOCK / 64) * R == TY, "waves tile the y rows");
+  static constexpr int NPP = (DYZ + NTH - 1) / NTH;     // y-z plane positions per thread
+  static constexpr int MAX_NCC = 2;                    // channels staged at once
+  // waves per SIMD asked of the register allocator (the weights of four
+  // samples need ~110 VGPRs)
+  static constexpr int MINW = BIG ? 4 : 1;
+  static_assert(TX * TY * TZ == NTH * R, "R samples per thread");
+  static_assert(WX * RX == TX && WX * WY == NW, "waves tile the x rows");
+  static_assert(SCAT ? WY * RY == TY : WY * R == TY, "waves tile the y rows");
 };
 
 // First sample of this thread within its tile (sample i adds i * (SDY, SDZ)).
-// Scatters: y-row fastest within a wave, then x-row.
 template <class B>
 __device__ __forceinline__ void tl_thread(int &sx, int &sy, int &sz0) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -90,7 +99,7 @@ __device__ __forceinline__ void tl_thread(int &sx, int &sy, int &sz0) {
     sy = row % B::RY + B::RY * (w / B::WX);
   } else {
     sx = row % B::RX + B::RX * (w % B::WX);
-    sy = w;
+    sy = w / B::WX;
   }
 }
 
@@ -106,7 +115,7 @@ struct TilePlane {
     const float r2 = 1.f / (float)E2;
 #pragma unroll
     for (int j = 0; j < B::NPP; ++j) {
-      const int p = threadIdx.x + j * NI_BLOCK;
+      const int p = threadIdx.x + j * B::NTH;
       int by = (int)((float)p * r2);   // floor(p / E2): exact after the fix-ups (p < 2^22)
       by -= by * E2 > p;
       by += (by + 1) * E2 <= p;
@@ -229,7 +238,8 @@ struct TileFlags {
 // NCC: channels per chunk (compile time, so that the channel loop of a tap
 // is branch-free; lanes of a partial last chunk compute unused channels).
 template <int ORD, int MODE, int NCC>
-__global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx, int nty, int ntz,
+__global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
+                             (TileBox<ORD, !TileMeta<MODE>::gather>::MINW)) void tile_kernel(PPArgs<float> a, int ntx, int nty, int ntz,
                                                         int64_t nblocks, int *fb) {
   constexpr int NT = ORD + 1;
   constexpr bool GATHER = TileMeta<MODE>::gather, DERIV = TileMeta<MODE>::deriv;
@@ -237,8 +247,9 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
   constexpr int R = B::R;
   constexpr int DY = B::DY, DZ = B::DZ, DYZ = B::DYZP, VOL = B::VOL;
   static_assert(NCC >= 1 && NCC <= B::MAX_NCC, "channel chunk exceeds the LDS budget");
+  constexpr int NTH = B::NTH;
   __shared__ float box[NCC * VOL];
-  __shared__ int red[NI_BLOCK / 64][6];
+  __shared__ int red[B::NW][6];
 
   const int64_t tile = xcd_block(nblocks);
   int64_t r = tile;
@@ -315,7 +326,7 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
     mn[d] = red[0][d];
     mx[d] = red[0][3 + d];
 #pragma unroll
-    for (int wv = 1; wv < NI_BLOCK / 64; ++wv) {
+    for (int wv = 1; wv < B::NW; ++wv) {
       mn[d] = min(mn[d], red[wv][d]);
       mx[d] = max(mx[d], red[wv][3 + d]);
     }
@@ -365,7 +376,7 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
 #pragma unroll
     for (int i = 0; i < R; ++i) n_in += __syncthreads_count(inb[i]);
   }
-  __shared__ float s_red[NI_BLOCK / 64][NCC];
+  __shared__ float s_red[B::NW][NCC];
   __shared__ float s_scale[NCC], s_inv[NCC];
 
   // folded offsets (in elements of the staged / scattered tensor) and signs
@@ -376,7 +387,7 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
   {
     const int64_t *st = GATHER ? a.src_s : a.out_s;
     const int st0 = (int)st[2], st1 = (int)st[3], st2 = (int)st[4];
-    for (int i = threadIdx.x; i < NF; i += NI_BLOCK) {
+    for (int i = threadIdx.x; i < NF; i += NTH) {
       const int d = i < B::DX ? 0 : i < B::DX + DY ? 1 : 2;
       const int b = i - (d == 0 ? 0 : d == 1 ? B::DX : B::DX + DY);
       const int e = d == 0 ? ext[0] : d == 1 ? ext[1] : ext[2];
@@ -394,7 +405,7 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
   pl.init(ext[1], ext[2], fofs, fsg);
   if (!GATHER) {   // scatter box: cleared once; each flush clears what it read
     int *ibox = (int *)box;
-    for (int i = threadIdx.x; i < NCC * VOL; i += NI_BLOCK) ibox[i] = 0;
+    for (int i = threadIdx.x; i < NCC * VOL; i += NTH) ibox[i] = 0;
     __syncthreads();
   }
 
@@ -477,7 +488,7 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
       if (threadIdx.x < NCC) {
         float m = s_red[0][threadIdx.x];
 #pragma unroll
-        for (int wv = 1; wv < NI_BLOCK / 64; ++wv) m = fmaxf(m, s_red[wv][threadIdx.x]);
+        for (int wv = 1; wv < B::NW; ++wv) m = fmaxf(m, s_red[wv][threadIdx.x]);
         int e = 0;
         if (m > 0.f && n_in > 0) {
           int q, kb = 0;
@@ -749,7 +760,7 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_kernel(PPArgs<float> a, int ntx
 // of the two kernels).  Kept apart from tile_kernel so that its register
 // demand does not lower that kernel's occupancy.
 template <int ORD, int MODE>
-__global__ __launch_bounds__(NI_BLOCK) void tile_fallback(PPArgs<float> a, int ntx, int nty, int ntz,
+__global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH)) void tile_fallback(PPArgs<float> a, int ntx, int nty, int ntz,
                                                           int64_t nblocks, const int *fb) {
   constexpr int NT = ORD + 1;
   constexpr int FL = TileFlags<MODE>::value;
@@ -774,7 +785,7 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_fallback(PPArgs<float> a, int n
     }
     return;
   }
-  __shared__ int red[NI_BLOCK / 64][6];
+  __shared__ int red[B::NW][6];
   int64_t r = xcd_block(nblocks);
   const int tz = (int)(r % ntz); r /= ntz;
   const int ty = (int)(r % nty); r /= nty;
@@ -827,7 +838,7 @@ __global__ __launch_bounds__(NI_BLOCK) void tile_fallback(PPArgs<float> a, int n
   for (int d = 0; d < 3; ++d) {
     int m0 = red[0][d], m1 = red[0][3 + d];
 #pragma unroll
-    for (int wv = 1; wv < NI_BLOCK / 64; ++wv) {
+    for (int wv = 1; wv < B::NW; ++wv) {
       m0 = min(m0, red[wv][d]);
       m1 = max(m1, red[wv][3 + d]);
     }
@@ -851,13 +862,13 @@ static void launch_tile(const PPArgs<float> &a, hipStream_t st) {
   const int nty = (int)((a.tY + B::TY - 1) / B::TY);
   const int ntz = (int)((a.tZ + B::TZ - 1) / B::TZ);
   const int64_t nb = a.N * (int64_t)ntx * nty * ntz;
-  const dim3 g((unsigned)nb), b(NI_BLOCK);
+  const dim3 g((unsigned)nb), b(B::NTH);
   int *fb = nullptr;
   if (hipMallocAsync((void **)&fb, sizeof(int) * (size_t)(nb + 1), st) != hipSuccess) {
     (void)hipGetLastError();
     fb = nullptr;
   } else {
-    hipLaunchKernelGGL(fill_zero<int>, dim3(1), b, 0, st, fb, (int64_t)1);
+    hipLaunchKernelGGL(fill_zero<int>, dim3(1), dim3(NI_BLOCK), 0, st, fb, (int64_t)1);
   }
   if constexpr (MODE == TM_COUNT) {
     hipLaunchKernelGGL((tile_kernel<ORD, MODE, 1>), g, b, 0, st, a, ntx, nty, ntz, nb, fb);
