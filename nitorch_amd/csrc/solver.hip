@@ -46,7 +46,7 @@
 namespace ni_amd {
 
 #define SV_BLOCK 256
-#define SV_SMALL_BYTES 49152    // levels whose solution (3 N V values) fits: single-block relax in LDS
+#define SV_SMALL_BYTES 49152    // LDS budget of the single-block relax when the device reports no more
 
 // Logical block id: the hardware dispatches consecutive blocks round-robin
 // over the 8 XCDs of an MI355X, each with its own L2; remapping gives every
@@ -346,9 +346,9 @@ __global__ __launch_bounds__(SV_BLOCK) void relax_colour(T *__restrict__ x, cons
 // All sweeps of a small level in one work-group, with a barrier between
 // classes; threads enumerate the members of each class directly.
 // Small levels: one work-group runs every sweep and colour with barriers in
-// between, on a copy of the solution held in LDS (3 N V values, at most
-// SV_SMALL_BYTES), so the 75 stencil reads per voxel and the updates never
-// leave the CU.  32-bit index arithmetic (these levels are small).
+// between, on a copy of the solution held in LDS (3 N V values, up to the
+// device's LDS per work-group), so the 75 stencil reads per voxel and the
+// updates never leave the CU.  32-bit index arithmetic (these levels are small).
 template <typename T>
 __global__ __launch_bounds__(1024) void relax_small(T *__restrict__ x, const T *__restrict__ g, const T *__restrict__ h,
                                                     int CC, Geo G, StencilT<T> S, int N, ColourPlan P, int nb_iter) {
@@ -757,13 +757,33 @@ static ColourPlan make_plan(const Geo &G, int band) {
   return P;
 }
 
+// Dynamic LDS one work-group of relax_small may use on the current device
+// (MI355X: 160 KB), cached per device; SV_SMALL_BYTES if the query fails.
+template <typename T>
+static size_t small_lds_bytes() {
+  static size_t cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return SV_SMALL_BYTES;
+  if (cache[dev]) return cache[dev];
+  int v = 0;
+  size_t lim = SV_SMALL_BYTES;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+      (size_t)v > lim &&
+      hipFuncSetAttribute((const void *)relax_small<T>, hipFuncAttributeMaxDynamicSharedMemorySize, v) ==
+          hipSuccess)
+    lim = (size_t)v;
+  (void)hipGetLastError();
+  cache[dev] = lim;
+  return lim;
+}
+
 template <typename T>
 static void launch_relax(T *x, const T *g, const T *h, int CC, const Geo &G, const StencilT<T> &S,
                          int N, int band, int nb_iter, hipStream_t st) {
   const long long V = (long long)G.X * G.Y * G.Z;
   const ColourPlan P = make_plan(G, band);
   const size_t xbytes = (size_t)3 * N * V * sizeof(T);
-  if (xbytes <= SV_SMALL_BYTES) {
+  if (xbytes <= small_lds_bytes<T>()) {
     hipLaunchKernelGGL(relax_small<T>, dim3(1), dim3(1024), xbytes, st, x, g, h, CC, G, S, N, P, nb_iter);
     return;
   }
