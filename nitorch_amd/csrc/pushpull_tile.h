@@ -80,9 +80,9 @@ struct TileBox {
   static constexpr int VOL = DX * DYZP;
   static constexpr int NPP = (DYZ + NTH - 1) / NTH;     // y-z plane positions per thread
   static constexpr int MAX_NCC = 2;                    // channels staged at once
-  // waves per SIMD asked of the register allocator (the weights of four
-  // samples need ~110 VGPRs)
-  static constexpr int MINW = BIG ? 4 : 1;
+  // waves per SIMD asked of the register allocator for the 512-thread
+  // scatter: 6 (<= 85 VGPRs) lets three 52 KB boxes share a CU's LDS
+  static constexpr int MINW = BIG ? 6 : 1;
   static_assert(TX * TY * TZ == NTH * R, "R samples per thread");
   static_assert(WX * RX == TX && WX * WY == NW, "waves tile the x rows");
   static_assert(SCAT ? WY * RY == TY : WY * R == TY, "waves tile the y rows");
@@ -266,10 +266,10 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
   tl_thread<B>(sx, sy, sz0);
   const int px = tx * B::TX + sx, py = ty * B::TY + sy, pz0 = tz * B::TZ + sz0;
   bool valid[R], inb[R];
-  // pull / push / count keep every sample's weights; gradient modes keep
-  // the coordinates and compute one sample's weights and derivatives at a
-  // time (registers)
-  constexpr int NW = DERIV ? 1 : R;
+  // pull keeps every sample's weights; the other modes keep the
+  // coordinates and compute one sample's weights (and derivatives) when it
+  // is processed (registers: occupancy)
+  constexpr int NW = (DERIV || !GATHER) ? 1 : R;
   int lo[R][3];
   float w[NW][3][NT];
   float xq[R][3];
@@ -296,7 +296,7 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
           float wtmp[NT], gtmp[NT];
           xq[i][d] = xs[d];
           lo[i][d] = tl_taps<ORD>(xs[d], wtmp, gtmp);
-          if constexpr (!DERIV) {
+          if constexpr (NW == R) {
 #pragma unroll
             for (int t = 0; t < NT; ++t) w[i][d][t] = wtmp[t];
           }
@@ -680,32 +680,54 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
         og[R - 1][2] = o2;
       }
     } else {
-      // push / count: fixed-point splats (exact: the scale is a power of two)
-      float sv[R][NCC];
-#pragma unroll
-      for (int i = 0; i < R; ++i)
-#pragma unroll
-        for (int c = 0; c < NCC; ++c) sv[i][c] = tv[i][c] * s_scale[c];
-      {
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-          if (!inb[i]) continue;
-          int *bp = (int *)box + off[i];
+      // push / count: fixed-point splats (exact: the scale is a power of
+      // two), one sample at a time (rolled): element 0 of the per-sample
+      // state is the current sample, rotated after each (R rotations
+      // restore it for the next channel chunk)
 #pragma unroll 1
-          for (int kx = 0; kx < NT; ++kx, bp += DYZ) {
+      for (int i = 0; i < R; ++i) {
+        const bool ii = inb[0];
+        const int oi = off[0];
+        const float q0 = xq[0][0], q1 = xq[0][1], q2 = xq[0][2];
+        float svi[NCC];
 #pragma unroll
-            for (int ky = 0; ky < NT; ++ky) {
-              const float wxy = w[i][0][0] * w[i][1][ky];
+        for (int c = 0; c < NCC; ++c) svi[c] = tv[0][c] * s_scale[c];
+        tl_rot(inb);
+        tl_rot(off);
 #pragma unroll
-              for (int kz = 0; kz < NT; ++kz) {
-                const float ww = wxy * w[i][2][kz];
-                const int o = ky * DZ + kz;
+        for (int d = 0; d < 3; ++d) {
+          const float t = xq[0][d];
 #pragma unroll
-                for (int c = 0; c < NCC; ++c) atomicAdd(bp + c * VOL + o, (int)rintf(ww * sv[i][c]));
-              }
+          for (int k = 0; k + 1 < R; ++k) xq[k][d] = xq[k + 1][d];
+          xq[R - 1][d] = t;
+        }
+#pragma unroll
+        for (int c = 0; c < NCC; ++c) {
+          const float t = tv[0][c];
+#pragma unroll
+          for (int k = 0; k + 1 < R; ++k) tv[k][c] = tv[k + 1][c];
+          tv[R - 1][c] = t;
+        }
+        if (!ii) continue;
+        float wi[3][NT], gi[3][NT];
+        tl_taps<ORD>(q0, wi[0], gi[0]);
+        tl_taps<ORD>(q1, wi[1], gi[1]);
+        tl_taps<ORD>(q2, wi[2], gi[2]);
+        int *bp = (int *)box + oi;
+#pragma unroll 1
+        for (int kx = 0; kx < NT; ++kx, bp += DYZ) {
+#pragma unroll
+          for (int ky = 0; ky < NT; ++ky) {
+            const float wxy = wi[0][0] * wi[1][ky];
+#pragma unroll
+            for (int kz = 0; kz < NT; ++kz) {
+              const float ww = wxy * wi[2][kz];
+              const int o = ky * DZ + kz;
+#pragma unroll
+              for (int c = 0; c < NCC; ++c) atomicAdd(bp + c * VOL + o, (int)rintf(ww * svi[c]));
             }
-            tl_rot(w[i][0]);
           }
+          tl_rot(wi[0]);
         }
       }
 
