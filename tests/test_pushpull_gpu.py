@@ -195,3 +195,35 @@ def test_config1_full_size():
     out = spatial.grid_pull(x.to(DEV), g.to(DEV), 1, "dct2", False)
     ref = O.grid_pull(x.numpy(), g.numpy(), "dct2", 1, 0)
     assert rel(out, ref) <= 1e-5
+
+
+def test_config3_directional_finite_differences():
+    """backward_128 recipe (order 3, dft, upstream gradient N(0,1)) generated
+    at 24^3, in float64: the grid and input gradients of L = <w, pull(x, g)>
+    against central differences along random directions (step 1e-4), and a
+    per-entry check on a few voxels.  extrapolate=True, so that no sample
+    crosses the out-of-bound cut within the step (the loss is smooth)."""
+    from nitorch_amd import grid as G
+    from nitorch_amd.bench_data import config3_inputs
+    x, g, w = (t.double().to(DEV).contiguous() for t in config3_inputs(batch=1, shape=(24, 24, 24)))
+    B, O, E = ["dft"], [3], 1
+
+    def L(x_, g_):
+        return float((G.grid_pull(x_, g_, B, O, E) * w).sum())
+
+    # outputs selected by requires_grad, as the reference's deque
+    gi, gg = G.grid_pull_backward(w, x.clone().requires_grad_(), g.clone().requires_grad_(), B, O, E)
+    rng = np.random.default_rng(0)
+    h = 1e-4
+    for _ in range(4):
+        vg = torch.from_numpy(rng.standard_normal(g.shape)).to(DEV)
+        vx = torch.from_numpy(rng.standard_normal(x.shape)).to(DEV)
+        fd_g = (L(x, g + h * vg) - L(x, g - h * vg)) / (2 * h)
+        fd_x = (L(x + h * vx, g) - L(x - h * vx, g)) / (2 * h)
+        assert abs(fd_g - float((gg * vg).sum())) <= 1e-6 * abs(fd_g) + 1e-8
+        assert abs(fd_x - float((gi * vx).sum())) <= 1e-8 * abs(fd_x) + 1e-8
+    for (i, j, k, d) in [(0, 0, 0, 0), (5, 11, 23, 1), (12, 3, 7, 2), (23, 23, 23, 0)]:
+        e = torch.zeros_like(g)
+        e[0, i, j, k, d] = 1
+        fd = (L(x, g + h * e) - L(x, g - h * e)) / (2 * h)
+        assert abs(fd - float(gg[0, i, j, k, d])) <= 1e-6 * max(1.0, abs(fd))
