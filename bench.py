@@ -150,6 +150,7 @@ def main():
     solve = None
     if not args.no_solve:
         solve = gn_solve(dev)
+        solve["svf_exp"] = svf_exp(dev, stream)
 
     cpu = None
     if not args.no_cpu:
@@ -254,6 +255,19 @@ def gn_solve(dev):
                           "method": "flexible PCG, U-Net (3 levels, 16/32/64, BF16 MFMA) warm start + "
                                     "preconditioner on block-Jacobi, FMG fallback"}
     return out
+
+
+def svf_exp(dev, stream):
+    """Scaling and squaring of a smooth 160x192x160x3 velocity (8 steps,
+    linear, dft): 8 pulls of the 3-channel field on the HIP kernels."""
+    import torch
+    from nitorch_amd import svf
+    from nitorch_amd.bench_data import smooth
+    gen = torch.Generator().manual_seed(5)
+    v = smooth(torch.randn((1, 3, 160, 192, 160), generator=gen, dtype=torch.float64), 4.0)
+    v = (v * (3.0 / v.abs().max())).movedim(1, -1).float().contiguous().to(dev)
+    ms, _ = timed(lambda: svf.exp(v, steps=8), 5, 1, stream)
+    return {"workload": "1x160x192x160x3 velocity, |v| <= 3 vox, 8 steps, linear, dft", "ms": round(ms, 3)}
 
 
 def cpu_baseline(x_cpu, g_cpu, slices):
