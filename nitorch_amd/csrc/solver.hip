@@ -120,36 +120,16 @@ struct Geo {
 // u: pointer to component 0 of this batch element; V = X*Y*Z.
 // INTERIOR: every position of the stencil is inside the volume (no folding,
 // all signs +1), which a wave checks before taking this path.
-template <typename T, bool INTERIOR>
-__device__ __forceinline__ void apply_L_t(const T *__restrict__ u, int V, const Geo &G,
-                                          const StencilT<T> &S, int x, int y, int z, T out[3]) {
-  // per-axis folded offsets and signs for displacements -2..2
-  int ox[5], oy[5], oz[5];
-  int sx[5], sy[5], sz[5];
-  const int sYZ = G.Y * G.Z, sZ = G.Z;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    if (INTERIOR) {
-      ox[k] = (x + k - 2) * sYZ; oy[k] = (y + k - 2) * sZ; oz[k] = z + k - 2;
-      sx[k] = sy[k] = sz[k] = 1;
-    } else {
-      ox[k] = fold<int>(G.b[0], x + k - 2, G.X, sx[k]) * sYZ;
-      oy[k] = fold<int>(G.b[1], y + k - 2, G.Y, sy[k]) * sZ;
-      oz[k] = fold<int>(G.b[2], z + k - 2, G.Z, sz[k]);
-    }
-  }
-  // value of component c at displacement (i, j, k) in {-2..2}, with sign
-  auto at = [&](int c, int i, int j, int k) -> T {
-    const T v = u[c * V + ox[i + 2] + oy[j + 2] + oz[k + 2]];
-    if (INTERIOR) return v;
-    const int s = sx[i + 2] * sy[j + 2] * sz[k + 2];
-    return s == 0 ? T(0) : (s < 0 ? -v : v);
-  };
+// (L u) of the three components at one voxel from an accessor
+// at(c, i, j, k) = signed value of component c at displacement (i, j, k),
+// |i|, |j|, |k| <= 2 (radius-2 axial, radius-1 diagonal positions only).
+template <typename T, class AT>
+__device__ __forceinline__ void apply_L_core(const AT &at, const StencilT<T> &S, T out[3]) {
   T val[3][13];   // per component: c, x-1, x+1, y-1, y+1, z-1, z+1, x-2, x+2, y-2, y+2, z-2, z+2
   T dxy[3][4], dxz[3][4], dyz[3][4];   // diagonals (--, +-, -+, ++)
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    val[c][0] = u[c * V + ox[2] + oy[2] + oz[2]];
+    val[c][0] = at(c, 0, 0, 0);
     val[c][1] = at(c, -1, 0, 0); val[c][2] = at(c, 1, 0, 0);
     val[c][3] = at(c, 0, -1, 0); val[c][4] = at(c, 0, 1, 0);
     val[c][5] = at(c, 0, 0, -1); val[c][6] = at(c, 0, 0, 1);
@@ -183,6 +163,40 @@ __device__ __forceinline__ void apply_L_t(const T *__restrict__ u, int V, const 
         + S.w002 * ((val[d][11] - c) + (val[d][12] - c));
     out[d] = lame + S.w2 * cross + bend * S.ivx[d];
   }
+}
+
+// Gather the 25 stencil positions of the three components around (x, y, z)
+// from global memory and return (L u) for each component.
+// u: pointer to component 0 of this batch element; V = X*Y*Z.
+// INTERIOR: every position of the stencil is inside the volume (no folding,
+// all signs +1), which a wave checks before taking this path.
+template <typename T, bool INTERIOR>
+__device__ __forceinline__ void apply_L_t(const T *__restrict__ u, int V, const Geo &G,
+                                          const StencilT<T> &S, int x, int y, int z, T out[3]) {
+  // per-axis folded offsets and signs for displacements -2..2
+  int ox[5], oy[5], oz[5];
+  int sx[5], sy[5], sz[5];
+  const int sYZ = G.Y * G.Z, sZ = G.Z;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    if (INTERIOR) {
+      ox[k] = (x + k - 2) * sYZ; oy[k] = (y + k - 2) * sZ; oz[k] = z + k - 2;
+      sx[k] = sy[k] = sz[k] = 1;
+    } else {
+      ox[k] = fold<int>(G.b[0], x + k - 2, G.X, sx[k]) * sYZ;
+      oy[k] = fold<int>(G.b[1], y + k - 2, G.Y, sy[k]) * sZ;
+      oz[k] = fold<int>(G.b[2], z + k - 2, G.Z, sz[k]);
+    }
+  }
+  // value of component c at displacement (i, j, k) in {-2..2}, with sign
+  // (the centre is always in the volume, sign +1)
+  auto at = [&](int c, int i, int j, int k) -> T {
+    const T v = u[c * V + ox[i + 2] + oy[j + 2] + oz[k + 2]];
+    if (INTERIOR || (i == 0 && j == 0 && k == 0)) return v;
+    const int s = sx[i + 2] * sy[j + 2] * sz[k + 2];
+    return s == 0 ? T(0) : (s < 0 ? -v : v);
+  };
+  apply_L_core<T>(at, S, out);
 }
 
 // (L u) at (x, y, z): the fold-free path when the whole wave is at least 2
@@ -349,8 +363,13 @@ __global__ __launch_bounds__(SV_BLOCK) void relax_colour(T *__restrict__ x, cons
 // between, on a copy of the solution held in LDS (3 N V values, up to the
 // device's LDS per work-group), so the 75 stencil reads per voxel and the
 // updates never leave the CU.  32-bit index arithmetic (these levels are small).
+// Double precision runs 512 threads, so that its stencil state fits the
+// registers of a 512-thread work-group without spilling to scratch.
 template <typename T>
-__global__ __launch_bounds__(1024) void relax_small(T *__restrict__ x, const T *__restrict__ g, const T *__restrict__ h,
+constexpr int relax_small_threads() { return sizeof(T) == 8 ? 512 : 1024; }
+
+template <typename T>
+__global__ __launch_bounds__(relax_small_threads<T>()) void relax_small(T *__restrict__ x, const T *__restrict__ g, const T *__restrict__ h,
                                                     int CC, Geo G, StencilT<T> S, int N, ColourPlan P, int nb_iter) {
   extern __shared__ char sv_smem[];
   T *xs = (T *)sv_smem;
@@ -544,6 +563,104 @@ __global__ __launch_bounds__(SV_BLOCK) void pcg_matvec(const T *__restrict__ p, 
     acc = (double)p0 * a0 + (double)p1 * a1 + (double)p2 * a2;
   }
   batch_reduce(acc, n, V, pap);
+}
+
+// LDS-tiled Ap = (H + L) p ; pap[n] += p.Ap for float fields.  A work-group
+// owns a 4 x 8 x 32 tile of voxels (one thread per (y, z) column of 4 x
+// voxels) and first stages the box of p around it -- the tile plus the
+// stencil radius 2 on every side, three components, folded by the boundary
+// condition and signed -- so each voxel's 75 stencil reads are LDS reads at
+// immediate offsets instead of scattered global loads.  Same formula as
+// pcg_matvec (apply_L_core); the p.Ap partial of the tile goes to one slot.
+#define MV_TX 4
+#define MV_TY 8
+#define MV_TZ 32
+#define MV_BX (MV_TX + 4)
+#define MV_BY (MV_TY + 4)
+#define MV_BZ (MV_TZ + 4)
+#define MV_BVOL (MV_BX * MV_BY * MV_BZ)
+static_assert(MV_TY * MV_TZ == SV_BLOCK, "one thread per (y, z) column of the tile");
+
+__global__ __launch_bounds__(SV_BLOCK) void pcg_matvec_tile(const float *__restrict__ p, const float *__restrict__ h,
+                                                            int CC, float *__restrict__ Ap, Geo G, StencilT<float> S,
+                                                            int ntx, int nty, int ntz, double *__restrict__ pap) {
+  __shared__ float box[3 * MV_BVOL];
+  __shared__ int fofs[MV_BX + MV_BY + MV_BZ];
+  __shared__ float fsg[MV_BX + MV_BY + MV_BZ];
+  __shared__ double s_acc[SV_BLOCK / 64];
+  const int V = G.X * G.Y * G.Z;
+  long long r = sv_block();
+  const int tz = (int)(r % ntz); r /= ntz;
+  const int ty = (int)(r % nty); r /= nty;
+  const int tx = (int)(r % ntx);
+  const int n = (int)(r / ntx);
+  const int x0 = tx * MV_TX, y0 = ty * MV_TY, z0 = tz * MV_TZ;
+  // folded element offset and sign of every box position, per axis
+  for (int i = threadIdx.x; i < MV_BX + MV_BY + MV_BZ; i += SV_BLOCK) {
+    const int d = i < MV_BX ? 0 : (i < MV_BX + MV_BY ? 1 : 2);
+    const int b = i - (d == 0 ? 0 : (d == 1 ? MV_BX : MV_BX + MV_BY));
+    const int org = d == 0 ? x0 : (d == 1 ? y0 : z0);
+    const int nn = d == 0 ? G.X : (d == 1 ? G.Y : G.Z);
+    const int st = d == 0 ? G.Y * G.Z : (d == 1 ? G.Z : 1);
+    int sg = 1;
+    const int j = fold<int>(G.b[d], org - 2 + b, nn, sg);
+    fofs[i] = j * st;
+    fsg[i] = (float)sg;
+  }
+  __syncthreads();
+  // stage the box: each thread takes (y, z) positions of the box plane and
+  // loads whole x rows (folded offsets are always in range)
+  const float *pn = p + (size_t)n * 3 * V;
+  for (int q = threadIdx.x; q < MV_BY * MV_BZ; q += SV_BLOCK) {
+    const int by = q / MV_BZ, bz = q - by * MV_BZ;
+    const int yzo = fofs[MV_BX + by] + fofs[MV_BX + MV_BY + bz];
+    const float yzs = fsg[MV_BX + by] * fsg[MV_BX + MV_BY + bz];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float v[MV_BX];
+#pragma unroll
+      for (int bx = 0; bx < MV_BX; ++bx) v[bx] = pn[c * V + fofs[bx] + yzo];
+#pragma unroll
+      for (int bx = 0; bx < MV_BX; ++bx) {
+        const float sg = fsg[bx] * yzs;
+        box[c * MV_BVOL + (bx * MV_BY + by) * MV_BZ + bz] = sg != 0.f ? v[bx] * sg : 0.f;
+      }
+    }
+  }
+  __syncthreads();
+  const int ly = threadIdx.x / MV_TZ, lz = threadIdx.x % MV_TZ;
+  const int y = y0 + ly, z = z0 + lz;
+  const float *hn = h ? h + (size_t)n * CC * V : nullptr;
+  float *An = Ap + (size_t)n * 3 * V;
+  double acc = 0.0;
+#pragma unroll 1
+  for (int lx = 0; lx < MV_TX; ++lx) {
+    const int x = x0 + lx;
+    if (x >= G.X || y >= G.Y || z >= G.Z) continue;
+    const float *bc = box + ((lx + 2) * MV_BY + (ly + 2)) * MV_BZ + (lz + 2);
+    auto at = [&](int c, int i, int j, int k) -> float { return bc[c * MV_BVOL + (i * MV_BY + j) * MV_BZ + k]; };
+    float Lp[3];
+    apply_L_core<float>(at, S, Lp);
+    const int iv = (x * G.Y + y) * G.Z + z;
+    float M[6];
+    load_H<float>(hn, hn ? CC : 0, V, iv, M);
+    const float p0 = bc[0], p1 = bc[MV_BVOL], p2 = bc[2 * MV_BVOL];
+    const float a0 = Lp[0] + M[0] * p0 + M[3] * p1 + M[4] * p2;
+    const float a1 = Lp[1] + M[3] * p0 + M[1] * p1 + M[5] * p2;
+    const float a2 = Lp[2] + M[4] * p0 + M[5] * p1 + M[2] * p2;
+    An[iv] = a0; An[V + iv] = a1; An[2 * V + iv] = a2;
+    acc += (double)p0 * a0 + (double)p1 * a1 + (double)p2 * a2;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) s_acc[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0;
+#pragma unroll
+    for (int k = 0; k < SV_BLOCK / 64; ++k) a += s_acc[k];
+    const int slot = (int)(((long long)(tx * nty + ty) * ntz + tz) % SV_SLOTS);
+    if (a != 0) atomicAdd(pap + (size_t)n * SV_SLOTS + slot, a);
+  }
 }
 
 // alpha = rz / max(pAp, 1e-12); x += alpha p; r -= alpha Ap; z = M \ r;
@@ -784,7 +901,7 @@ static void launch_relax(T *x, const T *g, const T *h, int CC, const Geo &G, con
   const ColourPlan P = make_plan(G, band);
   const size_t xbytes = (size_t)3 * N * V * sizeof(T);
   if (xbytes <= small_lds_bytes<T>()) {
-    hipLaunchKernelGGL(relax_small<T>, dim3(1), dim3(1024), xbytes, st, x, g, h, CC, G, S, N, P, nb_iter);
+    hipLaunchKernelGGL(relax_small<T>, dim3(1), dim3(relax_small_threads<T>()), xbytes, st, x, g, h, CC, G, S, N, P, nb_iter);
     return;
   }
   for (int it = 0; it < nb_iter; ++it) {
@@ -836,6 +953,21 @@ static size_t align_up(size_t b, size_t a) { return (b + a - 1) / a * a; }
 // pcg_common.cpp:38-85.  tol == 0: nb_iter iterations, no host synchronisation;
 // tol > 0: each iteration reads p.Ap and r.z back and stops when
 // sum_n alpha (alpha pAp + 2 rz) < tol * N * V, before updating (as the reference).
+// Ap = (H + L) p on the launch stream: the LDS-tiled kernel for float, the
+// per-voxel one otherwise.
+template <typename T>
+static void launch_matvec(const T *p, const T *h, int CC, T *Ap, const Geo &G, const StencilT<T> &S, int N,
+                          double *pap, unsigned bv, hipStream_t st) {
+  if constexpr (std::is_same<T, float>::value) {
+    const int ntx = (G.X + MV_TX - 1) / MV_TX, nty = (G.Y + MV_TY - 1) / MV_TY, ntz = (G.Z + MV_TZ - 1) / MV_TZ;
+    const long long nb = (long long)N * ntx * nty * ntz;
+    hipLaunchKernelGGL(pcg_matvec_tile, dim3((unsigned)nb), dim3(SV_BLOCK), 0, st, p, h, CC, Ap, G, S, ntx, nty,
+                       ntz, pap);
+  } else {
+    hipLaunchKernelGGL(pcg_matvec<T>, dim3(bv), dim3(SV_BLOCK), 0, st, p, h, CC, Ap, G, S, N, pap);
+  }
+}
+
 template <typename T>
 static int run_pcg(T *x, const T *g, const T *h, int CC, const Geo &G, const StencilT<T> &S, int N, int nb_iter,
                    double tol, T *r, T *z, T *p, double *sc, hipStream_t st, int *iters) {
@@ -850,7 +982,7 @@ static int run_pcg(T *x, const T *g, const T *h, int CC, const Geo &G, const Ste
   std::vector<double> host(2 * q);
   for (; it < nb_iter; ++it) {
     double *cur = R[it % 3], *nxt = R[(it + 1) % 3], *spare = R[(it + 2) % 3];
-    hipLaunchKernelGGL(pcg_matvec<T>, dim3(bv), dim3(SV_BLOCK), 0, st, p, h, CC, z, G, S, N, pap);
+    launch_matvec<T>(p, h, CC, z, G, S, N, pap, bv, st);
     if (tol > 0) {
       NI_HIP_CHECK(hipMemcpyAsync(host.data(), pap, q * sizeof(double), hipMemcpyDeviceToHost, st));
       NI_HIP_CHECK(hipMemcpyAsync(host.data() + q, cur, q * sizeof(double), hipMemcpyDeviceToHost, st));
