@@ -130,6 +130,15 @@ def main():
                                    "hbm_frac": bps3 * S3 / (bwd_ms / 1e3) / 1e9 / HBM_PEAK_GBS}
     del x3, g3, d3
 
+    # prefilter_192: interpolating coefficients of the 4x8x192^3 image (cubic,
+    # dct2), in place; one HBM read + write of the volume per axis
+    xs = x.clone()
+    pre = lambda: G.spline_coeff_nd(xs, ["dct2"], [3], 3, True)
+    pre_ms, _ = timed(pre, 5, 1, stream)
+    ops["prefilter_o3_dct2"] = {"ms": pre_ms, "gvox_s": S / pre_ms / 1e6,
+                                "hbm_frac": 3 * 8 * C * S / (pre_ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+    del xs
+
     sweep = {}
     if not args.no_sweep:
         for order in (0, 1, 3, 5):

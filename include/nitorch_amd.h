@@ -135,6 +135,17 @@ int ni_grid_grad_backward(int32_t dtype, int32_t dim, const int32_t *bound, cons
                           const int64_t *grad_input_stride, void *grad_grid,
                           const int64_t *grad_grid_stride, void *stream);
 
+/* Interpolating B-spline coefficients, in place.  Replaces interpol.coeff's
+ * spline_coeff / spline_coeff_nd, which nitorch/_C/grid.py:4 imports and
+ * SplineCoeff / SplineCoeffND (nitorch/_C/grid.py:335-390) call (restated
+ * in the reference's nitorch/_C/_ts/coeff.py:251-344).  `data` is a
+ * contiguous [nbatch, *shape] tensor with `ndim` (1..8) filtered axes;
+ * `bound[d]` / `order[d]` per axis (one entry each, no padding).  Orders 0
+ * and 1 and length-1 axes are left unchanged; other axes need bound zero,
+ * dct1 (mirror rules), replicate, dct2 (half-sample rules) or dft. */
+int ni_spline_coeff(int32_t dtype, int64_t nbatch, int32_t ndim, const int64_t *shape, const int32_t *bound,
+                    const int32_t *order, void *data, void *stream);
+
 /* ------------------------------------------------------------------------
  * Displacement-field solver (3-D).  Fields are [N, 3, *shape] and Hessians
  * [N, CC, *shape], contiguous, with CC in {1, 3, 6} (6 = h00 h11 h22 h01 h02

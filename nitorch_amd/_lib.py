@@ -46,6 +46,7 @@ _SIGS = {
                                _P, _PI64, _P, _PI64, _P, _PI64, _P],
     "ni_grid_grad_backward": [_I32, _I32, _PI32, _PI32, _I32, _I64, _I64, _PI64, _PI64,
                               _P, _PI64, _P, _PI64, _P, _PI64, _P, _PI64, _P, _PI64, _P],
+    "ni_spline_coeff": [_I32, _I64, _I32, _PI64, _PI32, _PI32, _P, _P],
 }
 
 # optional symbols (solver / learned preconditioner), registered when present

@@ -465,3 +465,118 @@ class GridGrad(torch.autograd.Function):
         if gi is not None and gi.shape[0] != input.shape[0]:
             gi = gi.sum(0, keepdim=True)
         return gi, gg, None, None, None
+
+
+# --------------------------------------------------------------------------
+# Interpolating spline coefficients (prefilter).  Same entry points as the
+# ones nitorch/_C/grid.py:4 imports from interpol.coeff (restated in the
+# reference's nitorch/_C/_ts/coeff.py:278-344) and the autograd Functions
+# SplineCoeff / SplineCoeffND of nitorch/_C/grid.py:335-390.
+# --------------------------------------------------------------------------
+_COEFF_BOUNDS = (BoundType.zero, BoundType.dct1, BoundType.replicate, BoundType.dct2, BoundType.dft)
+
+
+def _coeff_run(x, nbatch, shape, bound, order):
+    """In-place prefilter of the contiguous tensor x = [nbatch, *shape] along
+    every axis of `shape` (bound / order: one entry per axis)."""
+    for n, b, o in zip(shape, bound, order):
+        if o >= 2 and n > 1 and b not in _COEFF_BOUNDS:
+            raise NotImplementedError(
+                f"spline_coeff: boundary condition {BoundType(b).name} is not supported "
+                "(zero, dct1, replicate, dct2 and dft are)")
+    if x.numel() == 0 or all(o < 2 for o in order):
+        return x
+    L.require_device(x)
+    rc = L.lib().ni_spline_coeff(L.dtype_code(x.dtype), int(nbatch), len(shape), L.i64arr(shape),
+                                 L.i32arr(bound), L.i32arr(order), L.ptr(x), L.stream_of(x))
+    L.check(rc, "spline_coeff")
+    return x
+
+
+def _coeff_target(inp, inplace):
+    """(contiguous working tensor, whether to copy it back into `inp`)."""
+    if inplace and inp.is_contiguous():
+        return inp, False
+    return inp.clone(memory_format=torch.contiguous_format), inplace
+
+
+def spline_coeff(inp, bound, order, dim=-1, inplace=False):
+    """Interpolating coefficients along one dimension (`interpol.coeff.
+    spline_coeff(inp, bound, order, dim, inplace)`, nitorch/_C/_ts/coeff.py:
+    278-306); bound and order are single codes (BoundType / 0..7)."""
+    bound = int(bound_to_nitorch(bound[0] if isinstance(bound, (list, tuple)) else bound, "int"))
+    order = int(inter_to_nitorch(order[0] if isinstance(order, (list, tuple)) else order, "int"))
+    if inp.dim() == 0:
+        return inp if inplace else inp.clone()
+    dim = dim % inp.dim()
+    x, copy_back = _coeff_target(inp, inplace)
+    outer = 1
+    for s in x.shape[:dim]:
+        outer *= s
+    inner = 1
+    for s in x.shape[dim + 1:]:
+        inner *= s
+    _coeff_run(x, outer, [x.shape[dim], inner], [bound, bound], [order, 0])
+    if copy_back:
+        inp.copy_(x)
+        return inp
+    return x
+
+
+def spline_coeff_nd(inp, bound, order, dim=None, inplace=False):
+    """Interpolating coefficients along the last `dim` dimensions, first to
+    last (`interpol.coeff.spline_coeff_nd`, nitorch/_C/_ts/coeff.py:309-344);
+    bound / order lists are padded by repeating their last entry."""
+    dim = inp.dim() if dim is None else int(dim)
+    bound = [int(bound_to_nitorch(b, "int")) for b in make_list(bound)]
+    order = [int(inter_to_nitorch(o, "int")) for o in make_list(order)]
+    bound = (bound + [bound[-1]] * dim)[:dim]
+    order = (order + [order[-1]] * dim)[:dim]
+    if dim == 0:
+        return inp if inplace else inp.clone()
+    if dim > 8:
+        raise ValueError(f"spline_coeff_nd: at most 8 dimensions are supported (got {dim})")
+    x, copy_back = _coeff_target(inp, inplace)
+    nbatch = 1
+    for s in x.shape[:x.dim() - dim]:
+        nbatch *= s
+    _coeff_run(x, nbatch, list(x.shape[x.dim() - dim:]), bound, order)
+    if copy_back:
+        inp.copy_(x)
+        return inp
+    return x
+
+
+class SplineCoeff(torch.autograd.Function):
+
+    @staticmethod
+    def forward(ctx, input, bound, interpolation, dim, inplace):
+        opt = (bound_to_nitorch(bound, as_type="int"), inter_to_nitorch(interpolation, as_type="int"),
+               dim, inplace)
+        output = spline_coeff(input, *opt)
+        if input.requires_grad:
+            ctx.opt = opt
+        return output
+
+    @staticmethod
+    def backward(ctx, grad):
+        # the reference treats the filter as symmetric: backward == forward
+        grad = spline_coeff(grad, *ctx.opt[:-1], inplace=False)
+        return grad, None, None, None, None
+
+
+class SplineCoeffND(torch.autograd.Function):
+
+    @staticmethod
+    def forward(ctx, input, bound, interpolation, dim, inplace):
+        opt = (bound_to_nitorch(make_list(bound), as_type="int"),
+               inter_to_nitorch(make_list(interpolation), as_type="int"), dim, inplace)
+        output = spline_coeff_nd(input, *opt)
+        if input.requires_grad:
+            ctx.opt = opt
+        return output
+
+    @staticmethod
+    def backward(ctx, grad):
+        grad = spline_coeff_nd(grad, *ctx.opt[:-1], inplace=False)
+        return grad, None, None, None, None

@@ -9,10 +9,11 @@ runs in the HIP backend (`nitorch_amd.grid`).
 """
 import torch
 
-from .grid import (GridPull, GridPush, GridCount, GridGrad, BoundType,
-                   InterpolationType, bound_to_nitorch, inter_to_nitorch)
+from .grid import (GridPull, GridPush, GridCount, GridGrad, SplineCoeff, SplineCoeffND,
+                   BoundType, InterpolationType, bound_to_nitorch, inter_to_nitorch)
 
 __all__ = ["grid_pull", "grid_push", "grid_count", "grid_grad",
+           "spline_coeff", "spline_coeff_nd",
            "identity_grid", "add_identity_grid", "add_identity_grid_",
            "sub_identity_grid", "sub_identity_grid_",
            "BoundType", "InterpolationType"]
@@ -68,13 +69,6 @@ def _postproc(out, shape_info, mode):
     return out.reshape([*batch, *channel, *spatial, *feat])
 
 
-def _no_prefilter(prefilter):
-    if prefilter:
-        raise NotImplementedError(
-            "prefilter=True (spline interpolating coefficients) is not provided by the "
-            "HIP backend; pass B-spline coefficients or prefilter=False")
-
-
 def grid_pull(input, grid, interpolation="linear", bound="zero",
               extrapolate=False, prefilter=False):
     """Sample an image with respect to a deformation field
@@ -85,21 +79,30 @@ def grid_pull(input, grid, interpolation="linear", bound="zero",
     returns (..., [channel], *outshape)
 
     Integer inputs are treated as label maps: each label is pulled as a soft
-    mask and the arg-max label is returned.
+    mask and the arg-max label is returned.  With `prefilter`, the input is
+    first turned into interpolating spline coefficients (`spline_coeff_nd`).
+    For label maps this keeps the reference's statement
+    `input = spline_coeff_nd(soft, ..., inplace=True)` (_grid.py:193-195):
+    the prefiltered mask of one label replaces `input` for the next labels.
     """
-    _no_prefilter(prefilter)
     grid, input, shape_info = _preproc(grid, input)
     batch, channel = input.shape[:2]
+    dim = grid.shape[-1]
 
     if not input.dtype.is_floating_point:
         out = input.new_zeros([batch, channel, *grid.shape[1:-1]])
         pmax = grid.new_zeros([batch, channel, *grid.shape[1:-1]])
         for label in input.unique():
             soft = (input == label).to(grid.dtype)
+            if prefilter:
+                input = spline_coeff_nd(soft, interpolation=interpolation, bound=bound, dim=dim,
+                                        inplace=True)
             soft = GridPull.apply(soft, grid, interpolation, bound, extrapolate)
             out[soft > pmax] = label
             pmax = torch.max(pmax, soft)
     else:
+        if prefilter:
+            input = spline_coeff_nd(input, interpolation=interpolation, bound=bound, dim=dim)
         out = GridPull.apply(input, grid, interpolation, bound, extrapolate)
     return _postproc(out, shape_info, mode="pull")
 
@@ -107,12 +110,15 @@ def grid_pull(input, grid, interpolation="linear", bound="zero",
 def grid_push(input, grid, shape=None, interpolation="linear", bound="zero",
               extrapolate=False, prefilter=False):
     """Splat an image with respect to a deformation field (adjoint of pull,
-    nitorch/spatial/_grid.py:208-251)."""
-    _no_prefilter(prefilter)
+    nitorch/spatial/_grid.py:208-251); with `prefilter`, the splatted image is
+    turned into interpolating spline coefficients."""
     grid, input, shape_info = _preproc(grid, input, mode="push")
+    dim = grid.shape[-1]
     if shape is None:
         shape = tuple(input.shape[2:])
     out = GridPush.apply(input, grid, shape, interpolation, bound, extrapolate)
+    if prefilter:
+        out = spline_coeff_nd(out, interpolation=interpolation, bound=bound, dim=dim, inplace=True)
     return _postproc(out, shape_info, mode="push")
 
 
@@ -129,11 +135,29 @@ def grid_count(grid, shape=None, interpolation="linear", bound="zero",
 def grid_grad(input, grid, interpolation="linear", bound="zero",
               extrapolate=False, prefilter=False):
     """Spatial gradients of the spline at the grid points
-    (nitorch/spatial/_grid.py:290-328); returns (..., [channel], *outshape, dim)."""
-    _no_prefilter(prefilter)
+    (nitorch/spatial/_grid.py:290-328); returns (..., [channel], *outshape, dim).
+    With `prefilter`, the input is first turned into interpolating spline
+    coefficients."""
     grid, input, shape_info = _preproc(grid, input)
+    dim = grid.shape[-1]
+    if prefilter:
+        input = spline_coeff_nd(input, interpolation=interpolation, bound=bound, dim=dim)
     out = GridGrad.apply(input, grid, interpolation, bound, extrapolate)
     return _postproc(out, shape_info, mode="grad")
+
+
+def spline_coeff(input, interpolation="linear", bound="dct2", dim=-1, inplace=False):
+    """Interpolating spline coefficients along one dimension
+    (nitorch/spatial/_grid.py:331-373): the B-spline coefficients whose
+    order-`interpolation` spline passes through the samples, under the
+    boundary condition `bound` (zero/dct1, replicate/dct2 or dft rules)."""
+    return SplineCoeff.apply(input, bound, interpolation, dim, inplace)
+
+
+def spline_coeff_nd(input, interpolation="linear", bound="dct2", dim=None, inplace=False):
+    """Interpolating spline coefficients along the last `dim` dimensions
+    (nitorch/spatial/_grid.py:376-421)."""
+    return SplineCoeffND.apply(input, bound, interpolation, dim, inplace)
 
 
 def identity_grid(shape, dtype=None, device=None, jitter=False):
