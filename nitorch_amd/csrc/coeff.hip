@@ -99,12 +99,14 @@ __device__ double coeff_initial(const LN &a, int n, double z, int m, int rule) {
   if (rule == CF_WRAP) {
     const int mm = m < n ? m : n;
     double acc = (double)a.get(0), zk = z;
+#pragma unroll 4
     for (int k = 1; k < mm; ++k) { acc += zk * (double)a.get(n - k); zk *= z; }
     return acc / (1. - zk);   // zk = z^mm
   }
   if (rule == CF_MIRROR) {
     if (m < n) {
       double acc = (double)a.get(0), zk = z;
+#pragma unroll 4
       for (int k = 1; k < m; ++k) { acc += zk * (double)a.get(k); zk *= z; }
       return acc;
     }
@@ -122,6 +124,7 @@ __device__ double coeff_initial(const LN &a, int n, double z, int m, int rule) {
   const bool full = n <= 2 * m + 2;
   const int kend = full ? n - 1 : (m + 1 < n - 1 ? m + 1 : n - 1);
   double zk = z;
+#pragma unroll 4
   for (int k = 1; k < kend; ++k) { acc += zk * (double)a.get(k); zk *= z; }
   if (full) {
     double zr = zn * z;   // z^(n+1) at k = n-2
@@ -137,6 +140,7 @@ __device__ double coeff_final(const LN &a, int n, double z, int m, int rule) {
   if (rule == CF_WRAP) {
     const int mm = m < n ? m : n;
     double acc = z * (double)a.get(n - 1), zk = z * z;
+#pragma unroll 4
     for (int j = 0; j < mm - 1; ++j) { acc += zk * (double)a.get(j); zk *= z; }
     return acc / (ipow(z, mm) - 1.);
   }
@@ -153,18 +157,36 @@ __device__ void coeff_line(const LN &a, int n, const CoeffPlan &P) {
     const T zt = (T)z;
     T prev = (T)coeff_initial<T>(a, n, z, P.m[q], P.rule);
     a.set(0, prev);
+    // unrolled so that the reads of the next samples are in flight while the
+    // recursion runs (they do not depend on it)
+#pragma unroll 8
     for (int k = 1; k < n; ++k) {
       prev = a.get(k) + zt * prev;
       a.set(k, prev);
     }
     prev = (T)coeff_final<T>(a, n, z, P.m[q], P.rule);
     a.set(n - 1, prev);
+#pragma unroll 8
     for (int k = n - 2; k >= 0; --k) {
       prev = zt * (prev - a.get(k));
       a.set(k, prev);
     }
   }
 }
+
+// (line, k) = (e / n, e % n) of a lane's chunk position e = t + 64 j,
+// advanced by 64 per step without a division.
+struct CfWalk {
+  int line, k, q, r;
+  __device__ CfWalk(int t, int n) : line(t / n), k(t % n), q(CF_LINES / n), r(CF_LINES % n) {}
+  __device__ __forceinline__ void step(int n) {
+    k += r;
+    line += q;
+    const bool c = k >= n;
+    k -= c ? n : 0;
+    line += c;
+  }
+};
 
 // L lines of length n; line l starts at (l / I) * n * I + l % I with stride I.
 template <typename T>
@@ -178,17 +200,18 @@ __global__ __launch_bounds__(CF_LINES) void coeff_lds(T *__restrict__ data, long
   const T g = (T)P.gain;
   if (I == 1) {
     // rows l0 .. l0 + nl - 1 are one contiguous chunk of nl * n values
-    T *src = data + l0 * n;
-    const int tot = nl * n;
-    int line = t / n, k = t - line * n;
-    for (int e = t; e < tot; e += CF_LINES) {
-      buf[k * CF_PAD + line] = src[e] * g;
-      for (k += CF_LINES; k >= n; k -= n) ++line;
+    const T *src = data + l0 * n;
+    const int cnt = (nl * n - t + CF_LINES - 1) / CF_LINES;
+    CfWalk w(t, n);
+#pragma unroll 16
+    for (int j = 0; j < cnt; ++j) {
+      buf[w.k * CF_PAD + w.line] = src[t + j * CF_LINES] * g;
+      w.step(n);
     }
   } else if (t < nl) {
     const long long l = l0 + t;
-    T *src = data + (l / I) * n * I + l % I;
-#pragma unroll 8
+    const T *src = data + (l / I) * n * I + l % I;
+#pragma unroll 32
     for (int k = 0; k < n; ++k) buf[k * CF_PAD + t] = src[(long long)k * I] * g;
   }
   __syncthreads();
@@ -196,16 +219,17 @@ __global__ __launch_bounds__(CF_LINES) void coeff_lds(T *__restrict__ data, long
   __syncthreads();
   if (I == 1) {
     T *dst = data + l0 * n;
-    const int tot = nl * n;
-    int line = t / n, k = t - line * n;
-    for (int e = t; e < tot; e += CF_LINES) {
-      dst[e] = buf[k * CF_PAD + line];
-      for (k += CF_LINES; k >= n; k -= n) ++line;
+    const int cnt = (nl * n - t + CF_LINES - 1) / CF_LINES;
+    CfWalk w(t, n);
+#pragma unroll 16
+    for (int j = 0; j < cnt; ++j) {
+      dst[t + j * CF_LINES] = buf[w.k * CF_PAD + w.line];
+      w.step(n);
     }
   } else if (t < nl) {
     const long long l = l0 + t;
     T *dst = data + (l / I) * n * I + l % I;
-#pragma unroll 8
+#pragma unroll 32
     for (int k = 0; k < n; ++k) dst[(long long)k * I] = buf[k * CF_PAD + t];
   }
 }
