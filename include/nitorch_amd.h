@@ -146,6 +146,23 @@ int ni_grid_grad_backward(int32_t dtype, int32_t dim, const int32_t *bound, cons
 int ni_spline_coeff(int32_t dtype, int64_t nbatch, int32_t ndim, const int64_t *shape, const int32_t *bound,
                     const int32_t *order, void *data, void *stream);
 
+/* Replaces ni::resize (nitorch/_C/src/resize.cpp:25-59; bound as `resize`,
+ * `prolongation`, `restriction` and their backwards, spatial.cpp:99-108).
+ * input [N, C, *in_shape], output [N, C, *out_shape], `dim` (1..3) spatial
+ * dims, both contiguous and caller-allocated.  `factor`, `bound`, `order`
+ * and `mode` (GridAlignType edge 0, center 1, first 2, last 3) hold 3
+ * entries.  source = do_adjoint ? output : input, target = the other; each
+ * target voxel w samples the source at scale * w + shift (prepare_affine,
+ * resize_common.cpp:1548-1567) without out-of-bound masking; the adjoint
+ * splats into the zeroed source.  With `accumulate` the generic kernels add
+ * into the given output (the isotropic nearest / linear, and 3-D quadratic,
+ * ones overwrite it, as the reference's).  `normalize` scales the result by
+ * the scales with the reference's fall-through product. */
+int ni_resize(int32_t dtype, int32_t dim, int64_t N, int64_t C, const int64_t *in_shape, const int64_t *out_shape,
+              const double *factor, const int32_t *bound, const int32_t *order, const int32_t *mode,
+              int32_t do_adjoint, int32_t normalize, int32_t accumulate, const void *input, void *output,
+              void *stream);
+
 /* ------------------------------------------------------------------------
  * Displacement-field solver (3-D).  Fields are [N, 3, *shape] and Hessians
  * [N, CC, *shape], contiguous, with CC in {1, 3, 6} (6 = h00 h11 h22 h01 h02

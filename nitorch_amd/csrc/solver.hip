@@ -745,7 +745,8 @@ struct Xfer {
   int Xi, Yi, Zi, Xo, Yo, Zo;
   double sc[3];
   int b[3];
-  float chscale[6];
+  float chscale[6];   // per-channel factors (FMG rescaling, at most 6 channels)
+  int scaled;         // 0: no per-channel factor (any number of channels)
 };
 
 __device__ inline double w_lin(double x) { x = fabs(x); return x < 1.0 ? 1.0 - x : 0.0; }
@@ -801,7 +802,7 @@ __global__ __launch_bounds__(SV_BLOCK) void transfer_kernel(const T *__restrict_
     }
     acc += accy * wt[0][a];
   }
-  acc *= T(P.chscale[nc % C]);
+  if (P.scaled) acc *= T(P.chscale[nc % C]);
   T *dst = out + (size_t)nc * Vo + o;
   if (accumulate) *dst += acc;
   else *dst = acc;
@@ -937,7 +938,8 @@ static void launch_transfer(const T *in, T *out, int N, int C, const int *si, co
     P.sc[d] = (double)si[d] / (double)so[d];
     P.b[d] = bound[d];
   }
-  for (int c = 0; c < 6; ++c) P.chscale[c] = chscale ? chscale[c] : 1.0f;
+  for (int c = 0; c < 6; ++c) P.chscale[c] = (chscale && c < C) ? chscale[c] : 1.0f;
+  P.scaled = chscale != nullptr;   // FMG callers pass C <= 6 factors
   const long long n = (long long)N * C * so[0] * so[1] * so[2];
   if (restrict_)
     hipLaunchKernelGGL((transfer_kernel<T, 4>), dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, in, out, N * C, C, P, accumulate);

@@ -13,7 +13,7 @@ from .grid import (GridPull, GridPush, GridCount, GridGrad, SplineCoeff, SplineC
                    BoundType, InterpolationType, bound_to_nitorch, inter_to_nitorch)
 
 __all__ = ["grid_pull", "grid_push", "grid_count", "grid_grad",
-           "spline_coeff", "spline_coeff_nd",
+           "spline_coeff", "spline_coeff_nd", "affine_grid", "resize", "resize_grid", "reslice",
            "identity_grid", "add_identity_grid", "add_identity_grid_",
            "sub_identity_grid", "sub_identity_grid_",
            "BoundType", "InterpolationType"]
@@ -198,3 +198,177 @@ def sub_identity_grid_(grid):
 
 def sub_identity_grid(grid):
     return sub_identity_grid_(grid.clone())
+
+
+# --------------------------------------------------------------------------
+# Resampling on affine lattices (nitorch/spatial/_grid.py:579-915): grids
+# built with torch on the device, sampled by grid_pull (prefiltered by
+# default, as in the reference).
+# --------------------------------------------------------------------------
+def _matvec(mat, vec):
+    return torch.matmul(mat, vec[..., None])[..., 0]
+
+
+def _pad_last(vals, n):
+    vals = list(vals)
+    return (vals + vals[-1:] * n)[:n]
+
+
+def affine_grid(mat, shape, jitter=False):
+    """Grid of the voxel coordinates mat @ [i, j, k, 1] of a lattice of
+    `shape` (nitorch/spatial/_grid.py:579-615): (*batch, *shape, dim)."""
+    mat = torch.as_tensor(mat)
+    shape = list(shape)
+    nb_dim = mat.shape[-1] - 1
+    if nb_dim != len(shape):
+        raise ValueError("Dimension of the affine matrix ({}) and shape ({}) "
+                         "are not the same.".format(nb_dim, len(shape)))
+    if mat.shape[-2] not in (nb_dim, nb_dim + 1):
+        raise ValueError("First argument should be matrces of shape (..., {0}, {1}) or "
+                         "(..., {1}, {1}) but got {2}.".format(nb_dim, nb_dim + 1, mat.shape))
+    batch_shape = mat.shape[:-2]
+    grid = identity_grid(shape, mat.dtype, mat.device, jitter=jitter)
+    if batch_shape:
+        grid = grid.reshape([1] * len(batch_shape) + list(grid.shape))
+        mat = mat.reshape(list(batch_shape) + [1] * nb_dim + list(mat.shape[-2:]))
+    return _matvec(mat[..., :nb_dim, :nb_dim], grid) + mat[..., :nb_dim, -1]
+
+
+def resize(image, factor=None, shape=None, affine=None, anchor="c", dim=None, *args, **kwargs):
+    """Resize an image by a factor or to a shape (nitorch/spatial/_grid.py:618-754).
+
+    anchor per dimension: 'c' (corner centres), 'e' (outer edges), 'f'
+    (first voxel), 'l' (last voxel); extra arguments go to grid_pull, with
+    prefilter=True by default.  Returns the resized image and, with
+    `affine`, its orientation matrix."""
+    from .affine import affine_resize
+    image = torch.as_tensor(image)
+    nb_dim = dim or (len(shape) if shape is not None else image.dim() - 2)
+    inshape = image.shape[-nb_dim:]
+    info = {"dtype": image.dtype, "device": image.device}
+    if torch.is_tensor(factor):
+        factor = factor.tolist()
+    factor = _pad_last(factor if isinstance(factor, (list, tuple)) else [factor or 0.], nb_dim)
+    factor = [float(f or 0.) for f in factor]
+    outshape = _pad_last(shape if shape is not None else [None], nb_dim)
+    anchor = [a[0].lower() for a in _pad_last(anchor if isinstance(anchor, (list, tuple)) else [anchor],
+                                               nb_dim)]
+    return_trf = kwargs.pop("_return_trf", False)
+
+    outshape = [o or max(1, int(i * f)) for i, o, f in zip(inshape, outshape, factor)]
+    if any(not s for s in outshape):
+        raise ValueError("Either factor or shape must be set in all dimensions")
+    factor = [f or o / i for o, i, f in zip(outshape, inshape, factor)]
+
+    # input coordinate = scale * output coordinate + shift, per dimension
+    lin, scales, shifts = [], [], []
+    for anch, f, inshp, outshp in zip(anchor, factor, inshape, outshape):
+        if inshp == 1 or outshp == 1:
+            anch = "e"
+        if anch == "c":
+            lin.append(torch.linspace(0, inshp - 1, outshp, **info))
+            scales.append((inshp - 1) / (outshp - 1))
+            shifts.append(0)
+        elif anch == "e":
+            scale = inshp / outshp
+            shift = 0.5 * (scale - 1)
+            lin.append(torch.arange(0., outshp, **info) * scale + shift)
+            scales.append(scale)
+            shifts.append(shift)
+        elif anch == "f":
+            lin.append(torch.arange(0., outshp, **info) / f)
+            scales.append(1 / f)
+            shifts.append(0)
+        elif anch == "l":
+            shift = (inshp - 1) - (outshp - 1) / f
+            lin.append(torch.arange(0., outshp, **info) / f + shift)
+            scales.append(1 / f)
+            shifts.append(shift)
+        else:
+            raise ValueError("Unknown anchor {}".format(anch))
+        if inshp == outshp == 1:
+            scales[-1] = 1 / f
+    grid = torch.stack(torch.meshgrid(*lin, indexing="ij"), dim=-1)
+
+    kwargs.setdefault("prefilter", True)
+    resized = grid_pull(image, grid, *args, **kwargs)
+
+    if affine is not None:
+        affine, _ = affine_resize(affine, inshape, factor, anchor, shape_out=outshape)
+        if return_trf:
+            return resized, affine, (scales, shifts)
+        return resized, affine
+    if return_trf:
+        return resized, (scales, shifts)
+    return resized
+
+
+def resize_grid(grid, factor=None, shape=None, type="grid", affine=None, *args, **kwargs):
+    """Resize a transformation (type 'grid') or displacement field and
+    express it in the resized voxel space (nitorch/spatial/_grid.py:757-858)."""
+    kwargs["_return_trf"] = True
+    kwargs.setdefault("prefilter", True)
+    ndim = grid.shape[-1]
+    spatial = grid.shape[-ndim - 1:-1]
+    batch = grid.shape[:-ndim - 1]
+    if len(batch) == 0:
+        grid = grid[None]
+    elif len(batch) > 1:
+        grid = grid.reshape([-1, *spatial, ndim])
+    if affine is not None:
+        affine = affine.expand([*batch, *affine.shape[-2:]])
+        affine = affine.reshape([-1, *affine.shape[-2:]])
+    grid = grid.movedim(-1, 1)
+    outputs = resize(grid, factor, shape, affine, *args, **kwargs)
+    if affine is not None:
+        grid, affine, (scales, shifts) = outputs
+    else:
+        grid, (scales, shifts) = outputs
+    grid = grid.movedim(1, -1)
+    # original = scale * resized + shift  ->  resized = (original - shift) / scale
+    comps = []
+    for d, (scl, shft) in enumerate(zip(scales, shifts)):
+        g1 = grid[..., d]
+        if type[0].lower() == "g":
+            g1 = g1 - shft
+        comps.append(g1 / scl)
+    grid = torch.stack(comps, -1)
+    grid = grid.reshape([*batch, *grid.shape[-ndim - 1:]])
+    if affine is not None:
+        affine = affine.reshape([*batch, *affine.shape[-2:]])
+        return grid, affine
+    return grid
+
+
+def reslice(image, affine, affine_to, shape_to=None, **kwargs):
+    """Resample an image into another space (shape + affine)
+    (nitorch/spatial/_grid.py:861-915); grid_pull options, prefilter=True
+    by default."""
+    from .affine import affine_lmdiv
+    kwargs.setdefault("prefilter", True)
+    image = torch.as_tensor(image)
+    backend = dict(dtype=image.dtype, device=image.device)
+    if not backend["dtype"].is_floating_point:
+        backend["dtype"] = torch.get_default_dtype()
+    affine = torch.as_tensor(affine).to(**backend)
+    affine_to = torch.as_tensor(affine_to).to(**backend)
+    dim = affine.shape[-1] - 1
+    batch = affine.shape[:-2]
+    channels = image.shape[len(batch):-dim]
+    shape = image.shape[-dim:]
+    if shape_to is None:
+        shape_to = shape
+
+    def prod(s):
+        p = 1
+        for v in s:
+            p *= v
+        return p
+
+    transformation = affine_lmdiv(affine, affine_to)
+    grid = affine_grid(transformation, shape_to)
+    if not prod(batch):
+        grid = grid[None]
+    image = image.reshape([prod(batch) or 1, prod(channels) or 1, *shape])
+    image = grid_pull(image, grid, **kwargs)
+    return image.reshape([*batch, *channels, *shape_to])
