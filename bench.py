@@ -139,6 +139,15 @@ def main():
                                 "hbm_frac": 3 * 8 * C * S / (pre_ms / 1e3) / 1e9 / HBM_PEAK_GBS}
     del xs
 
+    # resize_192: the 4x8x192^3 image resized by 1/2 (cubic, edge-aligned,
+    # dct2) to 96^3; one read of the image and one write of the result
+    from nitorch_amd import resize as RZ
+    rz = lambda: RZ.resize(x, None, [0.5], ["dct2"], [3], ["edge"])
+    rz_ms, _ = timed(rz, 5, 1, stream)
+    vout = S // 8
+    ops["resize_half_o3_dct2"] = {"ms": rz_ms, "gvox_s": vout / rz_ms / 1e6,
+                                  "hbm_frac": 4 * C * (S + vout) / (rz_ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+
     sweep = {}
     if not args.no_sweep:
         for order in (0, 1, 3, 5):
