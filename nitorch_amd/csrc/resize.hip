@@ -23,6 +23,7 @@
 // into) one contiguous run; per-axis taps, weights, signs and folded
 // offsets are computed once per lane and reused over the channels.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 
 #include "common.h"
@@ -140,6 +141,153 @@ __global__ __launch_bounds__(RS_BLOCK) void resize_kernel(const T *__restrict__ 
   }
 }
 
+// One axis of the separable forward resize: in [A, nin, B] -> out [A, nout, B],
+// out[a, j, b] = (accumulate ? out : 0) + sum_t w_t(j) s_t in[a, fold(lo_j + t), b].
+// Lanes run along b (contiguous), so every tap is a coalesced row read.
+template <typename T>
+__global__ __launch_bounds__(RS_BLOCK) void resize_axis(const T *__restrict__ in, T *__restrict__ out,
+                                                        long long A, long long B, RsArgs a, int d,
+                                                        int accumulate) {
+  const int nout = a.nt[d], nin = a.ns[d];
+  const long long gid = (long long)blockIdx.x * RS_BLOCK + threadIdx.x;
+  if (gid >= A * nout * B) return;
+  const long long b = gid % B;
+  const int j = (int)((gid / B) % nout);
+  const long long o = gid / (B * nout);
+  T w[RS_MAXT];
+  long long off[RS_MAXT];
+  int sg[RS_MAXT];
+  const int nt = rs_taps<T>(a, d, j, B, w, off, sg);
+  const T *src = in + o * nin * B + b;
+  T acc = accumulate ? out[gid] : T(0);
+  for (int t = 0; t < nt; ++t) {
+    if (sg[t]) {
+      const T v = src[off[t]];
+      acc += (sg[t] < 0 ? -v : v) * w[t];
+    }
+  }
+  out[gid] = acc;
+}
+
+// Tap table of one axis: for each target index j, RS_MAXT entries of folded
+// source index (-1: no tap or sign 0, which the reference never reads) and
+// weight times sign.
+template <typename T>
+__global__ __launch_bounds__(RS_BLOCK) void rs_table(RsArgs a, int d, T *__restrict__ tw, int *__restrict__ ti) {
+  const int j = blockIdx.x * RS_BLOCK + threadIdx.x;
+  if (j >= a.nt[d]) return;
+  T w[RS_MAXT];
+  long long off[RS_MAXT];
+  int sg[RS_MAXT];
+  const int nt = rs_taps<T>(a, d, j, 1, w, off, sg);
+#pragma unroll
+  for (int t = 0; t < RS_MAXT; ++t) {
+    const bool on = t < nt && sg[t] != 0;
+    tw[j * RS_MAXT + t] = on ? (sg[t] < 0 ? -w[t] : w[t]) : T(0);
+    ti[j * RS_MAXT + t] = on ? (int)off[t] : -1;
+  }
+}
+
+// The same pass for long rows (B >= 64): work-group row r = (a, j) is uniform,
+// so its taps are read once from the table (scalar loads); lanes cover b.
+template <typename T>
+__global__ __launch_bounds__(RS_BLOCK) void resize_rows(const T *__restrict__ in, T *__restrict__ out, int nbx,
+                                                        long long B, int nin, int nout, int ntap,
+                                                        const T *__restrict__ tw, const int *__restrict__ ti,
+                                                        int accumulate) {
+  const int row = (int)(blockIdx.x / (unsigned)nbx);
+  const long long b = (long long)(blockIdx.x - (unsigned)row * (unsigned)nbx) * RS_BLOCK + threadIdx.x;
+  if (b >= B) return;
+  const int j = row % nout;
+  const long long o = row / nout;
+  const T *src = in + o * nin * B + b;
+  T *dst = out + (long long)row * B + b;
+  T acc = accumulate ? *dst : T(0);
+  for (int t = 0; t < ntap; ++t) {
+    const int i = ti[j * RS_MAXT + t];
+    if (i >= 0) acc += src[(long long)i * B] * tw[j * RS_MAXT + t];
+  }
+  *dst = acc;
+}
+
+// Long rows with B % 4 == 0: four consecutive values per lane (16-byte
+// loads and stores for float).
+template <typename T>
+__global__ __launch_bounds__(RS_BLOCK) void resize_rows4(const T *__restrict__ in, T *__restrict__ out, int nbx,
+                                                         long long B, int nin, int nout, int ntap,
+                                                         const T *__restrict__ tw, const int *__restrict__ ti,
+                                                         int accumulate) {
+  using V = HIP_vector_type<T, 4>;
+  const int row = (int)(blockIdx.x / (unsigned)nbx);
+  const long long b = ((long long)(blockIdx.x - (unsigned)row * (unsigned)nbx) * RS_BLOCK + threadIdx.x) * 4;
+  if (b >= B) return;
+  const int j = row % nout;
+  const long long o = row / nout;
+  const T *src = in + o * nin * B + b;
+  V *dst = (V *)(out + (long long)row * B + b);
+  V acc = accumulate ? *dst : V(T(0));
+  for (int t = 0; t < ntap; ++t) {
+    const int i = ti[j * RS_MAXT + t];
+    if (i >= 0) {
+      const V v = *(const V *)(src + (long long)i * B);
+      const T w = tw[j * RS_MAXT + t];
+      acc.x += v.x * w; acc.y += v.y * w; acc.z += v.z * w; acc.w += v.w * w;
+    }
+  }
+  *dst = acc;
+}
+
+// The contiguous axis (B == 1): a work-group stages `rows` whole input rows
+// (one contiguous chunk) and the axis's tap table in LDS, then writes its
+// rows x nout outputs as one contiguous chunk.
+template <typename T>
+__global__ __launch_bounds__(RS_BLOCK) void resize_contig(const T *__restrict__ in, T *__restrict__ out,
+                                                          long long A, int rows, int nin, int nout, int ntap,
+                                                          const T *__restrict__ tw, const int *__restrict__ ti,
+                                                          int accumulate) {
+  extern __shared__ char rs_smem[];
+  T *lw = (T *)rs_smem;                                  // nout * RS_MAXT weights
+  int *li = (int *)(lw + (size_t)nout * RS_MAXT);        // nout * RS_MAXT indices
+  T *lx = (T *)(li + (size_t)nout * RS_MAXT + ((sizeof(T) == 8) ? ((nout * RS_MAXT) & 1) : 0));
+  const long long r0 = (long long)blockIdx.x * rows;
+  const int nr = (int)(A - r0 < rows ? A - r0 : rows);
+  for (int k = threadIdx.x; k < nout * RS_MAXT; k += RS_BLOCK) { lw[k] = tw[k]; li[k] = ti[k]; }
+  const T *src = in + r0 * nin;
+  for (int k = threadIdx.x; k < nr * nin; k += RS_BLOCK) lx[k] = src[k];
+  __syncthreads();
+  T *dst = out + r0 * nout;
+  for (int k = threadIdx.x; k < nr * nout; k += RS_BLOCK) {
+    const int r = k / nout, j = k - r * nout;
+    const T *x = lx + r * nin;
+    T acc = accumulate ? dst[k] : T(0);
+    for (int t = 0; t < ntap; ++t) {
+      const int i = li[j * RS_MAXT + t];
+      if (i >= 0) acc += x[i] * lw[j * RS_MAXT + t];
+    }
+    dst[k] = acc;
+  }
+}
+
+// Short rows (B < 64) with fewer than 2^31 outputs: 32-bit index math.
+template <typename T>
+__global__ __launch_bounds__(RS_BLOCK) void resize_short(const T *__restrict__ in, T *__restrict__ out, int total,
+                                                         int B, int nin, int nout, int ntap,
+                                                         const T *__restrict__ tw, const int *__restrict__ ti,
+                                                         int accumulate) {
+  const int gid = blockIdx.x * RS_BLOCK + threadIdx.x;
+  if (gid >= total) return;
+  int b = 0, r = gid;
+  if (B > 1) { b = gid % B; r = gid / B; }
+  const int j = r % nout, o = r / nout;
+  const T *src = in + ((long long)o * nin) * B + b;
+  T acc = accumulate ? out[gid] : T(0);
+  for (int t = 0; t < ntap; ++t) {
+    const int i = ti[j * RS_MAXT + t];
+    if (i >= 0) acc += src[(long long)i * B] * tw[j * RS_MAXT + t];
+  }
+  out[gid] = acc;
+}
+
 template <typename T>
 __global__ __launch_bounds__(RS_BLOCK) void rs_fill(T *__restrict__ p, long long n, T v) {
   const long long i = (long long)blockIdx.x * RS_BLOCK + threadIdx.x;
@@ -167,6 +315,87 @@ static void rs_affine(double f, double Ds, double Dt, int mode, double &shift, d
   }
 }
 
+// Forward resize as separable passes x -> y -> z (axes of size 1 on both
+// sides are skipped): each pass streams its input once with coalesced tap
+// rows, instead of one lane gathering (O+1)^3 taps per channel from three
+// strided axes.  Intermediates come from a stream-ordered allocation; false
+// if it fails (the caller then uses the direct kernel).
+template <typename T>
+static bool rs_separable(const T *input, T *output, const RsArgs &a, hipStream_t st) {
+  int axes[3], np = 0;
+  for (int d = 0; d < 3; ++d)
+    if (!(a.ns[d] == 1 && a.nt[d] == 1)) axes[np++] = d;
+  if (np == 0) return false;
+  const long long NC = (long long)a.N * a.C;
+  long long cur[3] = {a.ns[0], a.ns[1], a.ns[2]};
+  // sizes of the intermediates (after each pass but the last)
+  long long tmp_elems[2] = {0, 0};
+  {
+    long long c[3] = {a.ns[0], a.ns[1], a.ns[2]};
+    for (int p = 0; p + 1 < np; ++p) {
+      c[axes[p]] = a.nt[axes[p]];
+      tmp_elems[p % 2] = std::max(tmp_elems[p % 2], NC * c[0] * c[1] * c[2]);
+    }
+  }
+  // buffer: the two intermediates, then per pass a tap table of nt[d] x
+  // RS_MAXT weights and as many int indices
+  long long tab_w[3], tab_i[3], tot = tmp_elems[0] + tmp_elems[1];
+  for (int p = 0; p < np; ++p) {
+    tab_w[p] = tot;
+    tot += (long long)a.nt[axes[p]] * RS_MAXT;
+    tab_i[p] = tot;
+    tot += ((long long)a.nt[axes[p]] * RS_MAXT * sizeof(int) + sizeof(T) - 1) / sizeof(T);
+  }
+  T *buf = nullptr;
+  if (hipMallocAsync((void **)&buf, (size_t)tot * sizeof(T), st) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  T *tmp[2] = {buf, buf + tmp_elems[0]};
+  for (int p = 0; p < np; ++p)
+    hipLaunchKernelGGL(rs_table<T>, dim3(rs_blocks(a.nt[axes[p]])), dim3(RS_BLOCK), 0, st, a, axes[p],
+                       buf + tab_w[p], (int *)(buf + tab_i[p]));
+  const T *src = input;
+  for (int p = 0; p < np; ++p) {
+    const int d = axes[p];
+    long long A = NC, B = 1;
+    for (int e = 0; e < d; ++e) A *= cur[e];
+    for (int e = d + 1; e < 3; ++e) B *= cur[e];
+    const bool last = p + 1 == np;
+    T *dst = last ? output : tmp[p % 2];
+    const long long n = A * a.nt[d] * B;
+    const int acc = last ? a.accumulate : 0;
+    const long long nbx = (B + RS_BLOCK - 1) / RS_BLOCK;
+    const T *tw = buf + tab_w[p];
+    const int *ti = (const int *)(buf + tab_i[p]);
+    const int ntap = a.order[d] + 1;
+    const long long nbx4 = (B / 4 + RS_BLOCK - 1) / RS_BLOCK;
+    // contiguous axis: rows of nin inputs staged in LDS (about 16 KB of them)
+    const int crows = (int)std::max<long long>(1, std::min<long long>(A, 16384 / ((long long)a.ns[d] * sizeof(T))));
+    const size_t clds = (size_t)a.nt[d] * RS_MAXT * (sizeof(T) + sizeof(int)) + sizeof(T) +
+                        (size_t)crows * a.ns[d] * sizeof(T);
+    if (B >= 64 && B % 4 == 0 && A * a.nt[d] * nbx4 < (1LL << 31))
+      hipLaunchKernelGGL(resize_rows4<T>, dim3((unsigned)(A * a.nt[d] * nbx4)), dim3(RS_BLOCK), 0, st, src, dst,
+                         (int)nbx4, B, a.ns[d], a.nt[d], ntap, tw, ti, acc);
+    else if (B >= 64 && A * a.nt[d] * nbx < (1LL << 31))
+      hipLaunchKernelGGL(resize_rows<T>, dim3((unsigned)(A * a.nt[d] * nbx)), dim3(RS_BLOCK), 0, st, src, dst,
+                         (int)nbx, B, a.ns[d], a.nt[d], ntap, tw, ti, acc);
+    else if (B == 1 && clds <= 32768 && (A + crows - 1) / crows < (1LL << 31) &&
+             (long long)crows * std::max(a.ns[d], a.nt[d]) < (1LL << 30))
+      hipLaunchKernelGGL(resize_contig<T>, dim3((unsigned)((A + crows - 1) / crows)), dim3(RS_BLOCK), clds, st,
+                         src, dst, A, crows, a.ns[d], a.nt[d], ntap, tw, ti, acc);
+    else if (n < (1LL << 31) - RS_BLOCK)
+      hipLaunchKernelGGL(resize_short<T>, dim3(rs_blocks(n)), dim3(RS_BLOCK), 0, st, src, dst, (int)n, (int)B,
+                         a.ns[d], a.nt[d], ntap, tw, ti, acc);
+    else
+      hipLaunchKernelGGL(resize_axis<T>, dim3(rs_blocks(n)), dim3(RS_BLOCK), 0, st, src, dst, A, B, a, d, acc);
+    cur[d] = a.nt[d];
+    src = dst;
+  }
+  (void)hipFreeAsync(buf, st);
+  return true;
+}
+
 template <typename T>
 static void rs_launch(const T *input, T *output, const RsArgs &a, bool adjoint, hipStream_t st) {
   const long long Vt = (long long)a.nt[0] * a.nt[1] * a.nt[2];
@@ -175,7 +404,7 @@ static void rs_launch(const T *input, T *output, const RsArgs &a, bool adjoint, 
     const long long ns = (long long)a.N * a.C * a.ns[0] * a.ns[1] * a.ns[2];
     hipLaunchKernelGGL(rs_fill<T>, dim3(rs_blocks(ns)), dim3(RS_BLOCK), 0, st, output, ns, T(0));
     hipLaunchKernelGGL((resize_kernel<T, true>), dim3(rs_blocks(nthreads)), dim3(RS_BLOCK), 0, st, input, output, a);
-  } else {
+  } else if (!rs_separable<T>(input, output, a, st)) {
     hipLaunchKernelGGL((resize_kernel<T, false>), dim3(rs_blocks(nthreads)), dim3(RS_BLOCK), 0, st, input, output, a);
   }
 }

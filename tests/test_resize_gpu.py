@@ -46,6 +46,23 @@ def test_resize_vs_oracle(dt, order):
                     assert rel(out, ref) <= tol, (mode, f, adjoint, normalize)
 
 
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32])
+def test_resize_long_rows_vs_oracle(dt):
+    """Rows of 64 or more values per pass (the row-per-work-group kernel)."""
+    from nitorch_amd import resize as RZ
+    tol = 1e-10 if dt == torch.float64 else 2e-5
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal((1, 2, 12, 40, 70))
+    for order, mode, f in (([3], ["edge"], [1.5]), ([0], ["center"], [0.6]), ([5, 1, 2], ["first"], [0.8])):
+        for given in (False, True):
+            shp = RS.out_shape(x.shape[2:], f, False)
+            o = rng.standard_normal((1, 2) + tuple(shp)) if given else None
+            ref = RS.resize(x, f, ["dft", "dct2", "replicate"], order, [RS.mode_code(m) for m in mode], None, o)
+            out = RZ.resize(T(x, dt), None if o is None else T(o, dt), f, ["dft", "dct2", "replicate"], order,
+                            mode)
+            assert rel(out, ref) <= tol, (order, mode, f, given)
+
+
 def test_given_output_and_other_dims():
     from nitorch_amd import resize as RZ
     rng = np.random.default_rng(7)
