@@ -169,6 +169,15 @@ __global__ __launch_bounds__(RS_BLOCK) void resize_axis(const T *__restrict__ in
   out[gid] = acc;
 }
 
+// Logical block id with the blocks of each XCD contiguous: the hardware
+// deals consecutive block ids round-robin over the 8 XCDs, so output rows
+// that share input rows would otherwise fetch them through different L2s.
+__device__ __forceinline__ unsigned rs_xcd_block() {
+  const unsigned b = blockIdx.x, n = gridDim.x;
+  const unsigned xcd = b % 8u, local = b / 8u, q = n / 8u, r = n % 8u;
+  return xcd * q + (xcd < r ? xcd : r) + local;
+}
+
 // Tap table of one axis: for each target index j, RS_MAXT entries of folded
 // source index (-1: no tap or sign 0, which the reference never reads) and
 // weight times sign.
@@ -195,8 +204,9 @@ __global__ __launch_bounds__(RS_BLOCK) void resize_rows(const T *__restrict__ in
                                                         long long B, int nin, int nout, int ntap,
                                                         const T *__restrict__ tw, const int *__restrict__ ti,
                                                         int accumulate) {
-  const int row = (int)(blockIdx.x / (unsigned)nbx);
-  const long long b = (long long)(blockIdx.x - (unsigned)row * (unsigned)nbx) * RS_BLOCK + threadIdx.x;
+  const unsigned blk = rs_xcd_block();
+  const int row = (int)(blk / (unsigned)nbx);
+  const long long b = (long long)(blk - (unsigned)row * (unsigned)nbx) * RS_BLOCK + threadIdx.x;
   if (b >= B) return;
   const int j = row % nout;
   const long long o = row / nout;
@@ -218,8 +228,9 @@ __global__ __launch_bounds__(RS_BLOCK) void resize_rows4(const T *__restrict__ i
                                                          const T *__restrict__ tw, const int *__restrict__ ti,
                                                          int accumulate) {
   using V = HIP_vector_type<T, 4>;
-  const int row = (int)(blockIdx.x / (unsigned)nbx);
-  const long long b = ((long long)(blockIdx.x - (unsigned)row * (unsigned)nbx) * RS_BLOCK + threadIdx.x) * 4;
+  const unsigned blk = rs_xcd_block();
+  const int row = (int)(blk / (unsigned)nbx);
+  const long long b = ((long long)(blk - (unsigned)row * (unsigned)nbx) * RS_BLOCK + threadIdx.x) * 4;
   if (b >= B) return;
   const int j = row % nout;
   const long long o = row / nout;
