@@ -285,7 +285,10 @@ def svf_exp(dev, stream):
     v = smooth(torch.randn((1, 3, 160, 192, 160), generator=gen, dtype=torch.float64), 4.0)
     v = (v * (3.0 / v.abs().max())).movedim(1, -1).float().contiguous().to(dev)
     ms, _ = timed(lambda: svf.exp(v, steps=8), 5, 1, stream)
-    return {"workload": "1x160x192x160x3 velocity, |v| <= 3 vox, 8 steps, linear, dft", "ms": round(ms, 3)}
+    g = torch.randn(v.shape, generator=gen).to(dev)
+    bms, _ = timed(lambda: svf.exp_backward(v, g, steps=8, rotate_grad=True), 3, 1, stream)
+    return {"workload": "1x160x192x160x3 velocity, |v| <= 3 vox, 8 steps, linear, dft", "ms": round(ms, 3),
+            "backward_ms": round(bms, 3), "backward": "exp_backward(rotate_grad=True), the anagrad path"}
 
 
 def cpu_baseline(x_cpu, g_cpu, slices):

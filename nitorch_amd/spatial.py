@@ -14,6 +14,7 @@ from .grid import (GridPull, GridPush, GridCount, GridGrad, SplineCoeff, SplineC
 
 __all__ = ["grid_pull", "grid_push", "grid_count", "grid_grad",
            "spline_coeff", "spline_coeff_nd", "affine_grid", "resize", "resize_grid", "reslice",
+           "grid_jacobian",
            "identity_grid", "add_identity_grid", "add_identity_grid_",
            "sub_identity_grid", "sub_identity_grid_",
            "BoundType", "InterpolationType"]
@@ -372,3 +373,62 @@ def reslice(image, affine, affine_to, shape_to=None, **kwargs):
     image = image.reshape([prod(batch) or 1, prod(channels) or 1, *shape])
     image = grid_pull(image, grid, **kwargs)
     return image.reshape([*batch, *channels, *shape_to])
+
+
+def _diff_central(x, dims, bound="dct2", voxel_size=1):
+    """First-order central differences of x along `dims`, stacked last
+    (nitorch/spatial/_finite_differences.py:154-184, `diff(side='c')`):
+    (x[i+1] - x[i-1]) / (2 vs) inside, the bound's rule at both ends, and a
+    copy of x along an axis of length 1."""
+    vs = list(voxel_size) if isinstance(voxel_size, (list, tuple)) else [voxel_size]
+    vs = (vs + vs[-1:] * len(dims))[:len(dims)]
+    out = []
+    for d, v in zip(dims, vs):
+        xm = x.movedim(d, 0)
+        n = xm.shape[0]
+        if n == 1:
+            out.append(xm.clone().movedim(0, d))
+            continue
+        df = torch.empty_like(xm)
+        df[1:-1] = xm[2:] - xm[:-2]
+        if bound in ("dct2", "replicate"):
+            df[0] = xm[1] - xm[0]
+            df[-1] = xm[-1] - xm[-2]
+        elif bound == "dct1":
+            df[0] = 0
+            df[-1] = 0
+        elif bound == "dst2":
+            df[0] = xm[1] + xm[0]
+            df[-1] = -(xm[-1] + xm[-2])
+        elif bound in ("dst1", "zero"):
+            df[0] = xm[1]
+            df[-1] = -xm[-2]
+        elif bound == "dft":
+            df[0] = xm[1] - xm[-1]
+            df[-1] = xm[0] - xm[-2]
+        else:
+            raise ValueError("Unknown boundary type {}.".format(bound))
+        out.append(df.div_(2 * v if v != 1 else 2.).movedim(0, d))
+    return torch.stack(out, -1)
+
+
+def grid_jacobian(grid, sample=None, bound="dft", voxel_size=1, type="grid", add_identity=True,
+                  extrapolate=True):
+    """Jacobian of a transformation / displacement field
+    (nitorch/spatial/_grid.py:917-969): central finite differences, or the
+    spline derivatives of grid_grad at `sample`.  Returns (..., *spatial,
+    dim, dim) with jac[..., i, j] = d psi_i / d x_j."""
+    grid = torch.as_tensor(grid)
+    dim = grid.shape[-1]
+    shape = grid.shape[-dim - 1:-1]
+    if type == "grid":
+        grid = grid - identity_grid(shape, dtype=grid.dtype, device=grid.device)
+    if sample is None:
+        jac = _diff_central(grid, list(range(-dim - 1, -1)), bound=bound, voxel_size=voxel_size)
+    else:
+        grid = grid.movedim(-1, -dim - 1)
+        jac = grid_grad(grid, sample, bound=bound, extrapolate=extrapolate)
+        jac = jac.movedim(-dim - 2, -2)
+    if add_identity:
+        torch.diagonal(jac, 0, -1, -2).add_(1)
+    return jac

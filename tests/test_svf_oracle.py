@@ -51,3 +51,93 @@ def test_step_doubling():
     d = a - ident
     sq = d + E.pull_vel(d, ident + d, 3)
     assert np.allclose(b - ident, sq, atol=1e-12)
+
+
+# --- Jacobians and the analytic backward (oracle/svf.py) -------------------
+_PADS = {"dct2": "symmetric", "replicate": "edge", "dct1": "reflect", "dft": "wrap", "zero": "constant"}
+
+
+def test_central_differences_match_padded_definition():
+    """Each bound's end rule equals the central difference of the line
+    extended by numpy.pad's matching mode; dst2 by the antisymmetric
+    extension x[-1] = -x[0]."""
+    rng = np.random.default_rng(30)
+    x = rng.standard_normal((3, 7, 2))
+    for bound, mode in _PADS.items():
+        d = E.diff_central(x, [1], bound)[..., 0]
+        p = np.pad(x, ((0, 0), (1, 1), (0, 0)), mode=mode)
+        assert np.allclose(d, (p[:, 2:] - p[:, :-2]) / 2, atol=1e-14), bound
+    p = np.concatenate([-x[:, :1], x, -x[:, -1:]], 1)
+    assert np.allclose(E.diff_central(x, [1], "dst2")[..., 0], (p[:, 2:] - p[:, :-2]) / 2, atol=1e-14)
+
+
+def test_grid_jacobian_of_a_periodic_field():
+    """dft central difference of sin(2 pi k x / n) is cos(2 pi k x / n) sin(2 pi k / n);
+    jac[..., i, j] = d psi_i / d x_j."""
+    shape = (8, 6, 5)
+    idx = E.identity(shape)
+    k = 2 * np.pi * np.array([1, 2, 1]) / np.array(shape)
+    disp = np.zeros((1,) + shape + (3,))
+    disp[0, ..., 1] = np.sin(k[0] * idx[..., 0])          # psi_1 depends on x_0
+    jac = E.grid_jacobian(disp, bound="dft", type="disp")
+    assert np.allclose(jac[0, ..., 1, 0], np.cos(k[0] * idx[..., 0]) * np.sin(k[0]), atol=1e-12)
+    assert np.allclose(jac[0, ..., 0, 0], 1) and np.allclose(jac[0, ..., 1, 1], 1)
+    assert np.allclose(jac[0, ..., 0, 1], 0) and np.allclose(jac[0, ..., 2, 0], 0)
+    # a transformation field gives the same Jacobian as its displacement
+    assert np.allclose(E.grid_jacobian(disp + E.identity(shape)[None], bound="dft"), jac)
+
+
+def test_forward_jacobian_constant_and_smooth():
+    c = np.array([0.7, -1.3, 2.1])
+    v = np.broadcast_to(c, (1, 6, 7, 5, 3)).copy()
+    disp, jac = E.exp_forward_jac(v, steps=3)
+    assert np.allclose(jac, np.eye(3), atol=1e-12) and np.allclose(disp, c, atol=1e-12)
+    # smooth field: the composed Jacobian tracks the finite-difference one
+    v = _smooth_vel((24, 24, 24), 0.6, 3)
+    disp, jac = E.exp_forward_jac(v, steps=4)
+    fd = E.grid_jacobian(disp, bound="dft", type="disp")
+    assert np.abs(jac - fd).max() <= 0.1 * np.abs(fd - np.eye(3)).max()
+
+
+def test_backward_constant_velocity_closed_form():
+    """Constant velocity c with c / 2^s an integer step u (dft, linear):
+    grad_out(x) = 2^-s sum_{m < 2^s} grad(x - m u); inverse: the negated sum
+    over x + m u."""
+    rng = np.random.default_rng(31)
+    s, u = 2, np.array([1, 0, 2])
+    v = np.broadcast_to(u * 2.0 ** s, (1, 6, 5, 8, 3)).copy()
+    g = rng.standard_normal(v.shape)
+    ref = sum(np.roll(g, shift=tuple(m * u), axis=(1, 2, 3)) for m in range(2 ** s)) / 2 ** s
+    assert np.allclose(E.exp_backward(v, g, steps=s), ref, atol=1e-12)
+    ref_i = -sum(np.roll(g, shift=tuple(-m * u), axis=(1, 2, 3)) for m in range(2 ** s)) / 2 ** s
+    assert np.allclose(E.exp_backward(v, g, steps=s, inverse=True), ref_i, atol=1e-12)
+    # with a Hessian: J = I, det = 1, so it is summed the same way
+    h = rng.standard_normal(v.shape[:-1] + (6,))
+    _, hh = E.exp_backward(v, g, h, steps=s)
+    assert np.allclose(hh, sum(np.roll(h, shift=tuple(m * u), axis=(1, 2, 3)) for m in range(2 ** s)) / 2 ** s,
+                       atol=1e-12)
+
+
+def test_backward_tracks_finite_differences():
+    """For a small smooth velocity the analytic (DARTEL) gradient of
+    L(v) = <exp(v), g> agrees with a central finite difference to first order."""
+    rng = np.random.default_rng(32)
+    v = _smooth_vel((16, 16, 16), 0.3, 4)
+    g = rng.standard_normal(v.shape)
+    d = _smooth_vel((16, 16, 16), 1.0, 5)
+    eps = 1e-4
+    lp = (E.exp_forward(v + eps * d, steps=5) * g).sum()
+    lm = (E.exp_forward(v - eps * d, steps=5) * g).sum()
+    fd = (lp - lm) / (2 * eps)
+    an = (E.exp_backward(v, g, steps=5, rotate_grad=True) * d).sum()
+    assert abs(an - fd) <= 0.1 * abs(fd)
+
+
+def test_jhj_is_congruence():
+    rng = np.random.default_rng(33)
+    J = rng.standard_normal((4, 3, 3))
+    h = rng.standard_normal((4, 6))
+    H = E._sym(h)
+    out = E._sym(E.jhj(J, h))
+    for i in range(4):
+        assert np.allclose(out[i], J[i] @ H[i] @ J[i].T)
