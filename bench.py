@@ -10,7 +10,10 @@ HBM.  `value` = spatial samples processed per second over the step
 (2 x 4 x 192^3 samples per step), in Gvox/s.
 
 Also reported: per-op times (and the autograd backward of an order-3 dft
-pull on 2x3x128x160x128, both gradients), the order x bound sweep of pull/push/count/grad
+pull on 2x3x128x160x128, both gradients; the cubic dct2 prefilter of the
+4x8x192^3 image; its cubic edge-aligned resize to 96^3), the Gauss-Newton
+solves of the 160x192x160 system and the SVF exponentiation and its analytic
+backward on that lattice, the order x bound sweep of pull/push/count/grad
 (orders 0/1/3/5 x dct2/dft/replicate/zero, one timed launch each after a
 warm-up), the roofline of the dominant kernel (algorithmic bytes per launch /
 average launch time, HIP events on the launch stream), and the reference's
