@@ -227,3 +227,37 @@ def test_config3_directional_finite_differences():
         e[0, i, j, k, d] = 1
         fd = (L(x, g + h * e) - L(x, g - h * e)) / (2 * h)
         assert abs(fd - float(gg[0, i, j, k, d])) <= 1e-6 * max(1.0, abs(fd))
+
+
+@pytest.mark.parametrize("prefilter", [False, True])
+def test_label_map_pull_vs_oracle(prefilter):
+    """Integer inputs are pulled as one soft mask per label and the arg-max
+    label kept (nitorch/spatial/_grid.py:187-198), here against the same
+    procedure on the oracle's pull.  With `prefilter`, the reference assigns
+    the prefiltered mask to `input` (in place), so later labels are compared
+    with coefficients: reproduced."""
+    from nitorch_amd import spatial
+    from oracle import coeff as CF
+    rng = np.random.default_rng(17)
+    lab = rng.integers(0, 5, (2, 1, 9, 8, 7))
+    grid = (O_identity((9, 8, 7))[None] + rng.uniform(-1.5, 1.5, (2, 9, 8, 7, 3)))
+    order = 3 if prefilter else 1
+    out = spatial.grid_pull(torch.from_numpy(lab).to(DEV), torch.from_numpy(grid).to(DEV), interpolation=order,
+                            bound="dct2", extrapolate=False, prefilter=prefilter)
+    ref = np.zeros(lab.shape, dtype=lab.dtype)
+    pmax = np.zeros(lab.shape)
+    inp = lab
+    for label in np.unique(lab):
+        soft = (inp == label).astype(np.float64)
+        if prefilter:
+            soft = CF.spline_coeff_nd(soft, "dct2", order, dim=3)
+            inp = soft
+        s = O.grid_pull(soft, grid, "dct2", order, 0)
+        ref[s > pmax] = label
+        pmax = np.maximum(pmax, s)
+    assert out.dtype == torch.from_numpy(lab).dtype
+    assert np.array_equal(out.cpu().numpy(), ref)
+
+
+def O_identity(shape):
+    return np.stack(np.meshgrid(*[np.arange(s, dtype=np.float64) for s in shape], indexing="ij"), -1)
