@@ -321,6 +321,20 @@ def cpu_baseline(x_cpu, g_cpu, slices):
         m.grid_push(y, g, list(x.shape[2:]), B, I, 0)
         t2 = time.perf_counter()
         kind = "reference"
+        # the same on one host thread, batch 0, first 8 x-slices
+        g1 = g[:1, :8].contiguous()
+        y1 = y[:1, :, :8].contiguous()
+        S1 = g1[..., 0].numel()
+        torch.set_num_threads(1)
+        try:
+            u0 = time.perf_counter()
+            m.grid_pull(x[:1], g1, B, I, 0)
+            m.grid_push(y1, g1, list(x.shape[2:]), B, I, 0)
+            u1 = time.perf_counter()
+        finally:
+            torch.set_num_threads(threads)
+        single = {"value": round(2 * S1 / (u1 - u0) / 1e9, 6), "unit": "Gvox/s", "cores": 1,
+                  "sample": f"batch 0, first 8 x-slices ({S1} samples), pull + push"}
     else:
         from oracle import pushpull as O
         x = x[:1]
@@ -334,9 +348,10 @@ def cpu_baseline(x_cpu, g_cpu, slices):
         t2 = time.perf_counter()
         kind, threads = "port", 1
         sample = f"batch 0, first 4 x-slices ({S} samples), single-threaded oracle port"
+        single = None
     return {"value": round(2 * S / (t2 - t0) / 1e9, 5), "unit": "Gvox/s", "cores": threads,
             "kind": kind, "sample": sample,
-            "pull_s": round(t1 - t0, 3), "push_s": round(t2 - t1, 3)}
+            "pull_s": round(t1 - t0, 3), "push_s": round(t2 - t1, 3), "single_thread": single}
 
 
 if __name__ == "__main__":
