@@ -207,7 +207,9 @@ def sub_identity_grid(grid):
 # default, as in the reference).
 # --------------------------------------------------------------------------
 def _matvec(mat, vec):
-    return torch.matmul(mat, vec[..., None])[..., 0]
+    # broadcast multiply-and-sum: one elementwise kernel, not a batched GEMM
+    # over every voxel
+    return (mat * vec[..., None, :]).sum(-1)
 
 
 def _pad_last(vals, n):

@@ -81,11 +81,17 @@ def _composition_jac(jac, rhs, lhs=None, type="grid", identity=None, **kwargs):
             identity = SP.identity_grid(rhs.shape[-dim - 1:-1], dtype=rhs.dtype, device=rhs.device)
         rhs = rhs + identity
     jac_left = _pull_jac(jac_left, rhs)
-    return torch.matmul(jac_left, jac)
+    return _matmul(jac_left, jac)
 
 
+# Products of fields of small matrices as broadcast multiply-and-sum (one
+# elementwise kernel each) rather than batched GEMMs of millions of 3 x 3s.
 def _matvec(mat, vec):
-    return torch.matmul(mat, vec[..., None])[..., 0]
+    return (mat * vec[..., None, :]).sum(-1)
+
+
+def _matmul(a, b):
+    return (a[..., :, :, None] * b[..., None, :, :]).sum(-2)
 
 
 def _det(m):
@@ -126,7 +132,7 @@ def _jhj(jac, hess):
     H = torch.stack([torch.stack([hess[..., 0], hess[..., 3], hess[..., 4]], -1),
                      torch.stack([hess[..., 3], hess[..., 1], hess[..., 5]], -1),
                      torch.stack([hess[..., 4], hess[..., 5], hess[..., 2]], -1)], -2)
-    out = torch.matmul(torch.matmul(jac, H), jac.transpose(-1, -2))
+    out = _matmul(_matmul(jac, H), jac.transpose(-1, -2))
     return torch.stack([out[..., 0, 0], out[..., 1, 1], out[..., 2, 2],
                         out[..., 0, 1], out[..., 0, 2], out[..., 1, 2]], -1)
 
