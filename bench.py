@@ -151,6 +151,16 @@ def main():
     ops["resize_half_o3_dct2"] = {"ms": rz_ms, "gvox_s": vout / rz_ms / 1e6,
                                   "hbm_frac": 4 * C * (S + vout) / (rz_ms / 1e3) / 1e9 / HBM_PEAK_GBS}
 
+    # pull_96: the reference's CPU-runnable case, 1x1x96^3, order 1, dct2
+    from nitorch_amd.bench_data import config1_inputs
+    x1c, g1c = config1_inputs()
+    x1, g1 = x1c.to(dev), g1c.to(dev)
+    S1 = g1[..., 0].numel()
+    p1 = lambda: G.grid_pull(x1, g1, ["dct2"], [1], 0)
+    p1_ms, _ = timed(p1, max(args.steps, 5), 2, stream)
+    ops["pull_96_o1_dct2"] = {"ms": p1_ms, "gvox_s": S1 / p1_ms / 1e6,
+                              "hbm_frac": 20 * S1 / (p1_ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+
     sweep = {}
     if not args.no_sweep:
         for order in (0, 1, 3, 5):
@@ -176,6 +186,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(x_cpu, g_cpu, args.cpu_slices)
+        cpu["pull_96_o1_dct2"] = cpu_pull96(x1c, g1c)
 
     line = {
         "metric": METRIC,
@@ -352,6 +363,28 @@ def cpu_baseline(x_cpu, g_cpu, slices):
     return {"value": round(2 * S / (t2 - t0) / 1e9, 5), "unit": "Gvox/s", "cores": threads,
             "kind": kind, "sample": sample,
             "pull_s": round(t1 - t0, 3), "push_s": round(t2 - t1, 3), "single_thread": single}
+
+
+def cpu_pull96(x, g):
+    """The reference CPU backend on the whole pull_96 case (median of 5)."""
+    import torch
+    try:
+        from oracle import ref as R
+        m = R.load()
+    except Exception:
+        m = None
+    if m is None:
+        return None
+    B, I = R.bounds(["dct2"]), R.orders([1])
+    m.grid_pull(x, g, B, I, 0)
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        m.grid_pull(x, g, B, I, 0)
+        ts.append(time.perf_counter() - t0)
+    ms = sorted(ts)[2] * 1e3
+    return {"ms": round(ms, 3), "gvox_s": round(g[..., 0].numel() / ms / 1e6, 4), "cores": torch.get_num_threads(),
+            "kind": "reference"}
 
 
 if __name__ == "__main__":
