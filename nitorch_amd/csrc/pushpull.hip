@@ -96,6 +96,20 @@ static int run_pushpull(PPArgs<T> a, hipStream_t st) {
   }
   if (total == 0 || a.C == 0) return 0;
 
+  // grid_grad of nearest-neighbour splines is zero everywhere (the reference
+  // zero-fills it, pushpull_common.cpp:513-516): a contiguous output is
+  // cleared without sampling anything
+  if (a.flags == F_SGRAD && a.order[0] == 0 && (a.dim < 2 || a.order[1] == 0) && (a.dim < 3 || a.order[2] == 0)) {
+    const int64_t osz[6] = {a.N, a.C, a.tX, a.tY, a.tZ, a.dim};
+    bool contig = true;
+    int64_t expect = 1;
+    for (int k = 5; k >= 0; --k) {
+      if (osz[k] > 1 && a.out_s[k] != expect) contig = false;
+      expect *= osz[k];
+    }
+    if (contig) return zero_fill<T>(a.out, expect, st);
+  }
+
   // 32-bit offsets when every tensor allows it
   const int64_t ssz[5] = {a.N, a.C, a.sX, a.sY, a.sZ};
   const int64_t tsz[6] = {a.N, a.C, a.tX, a.tY, a.tZ, (a.flags & F_KTRGT) ? a.dim : 1};

@@ -385,7 +385,10 @@ static bool rs_separable(const T *input, T *output, const RsArgs &a, hipStream_t
     const int crows = (int)std::max<long long>(1, std::min<long long>(A, 16384 / ((long long)a.ns[d] * sizeof(T))));
     const size_t clds = (size_t)a.nt[d] * RS_MAXT * (sizeof(T) + sizeof(int)) + sizeof(T) +
                         (size_t)crows * a.ns[d] * sizeof(T);
-    if (B >= 64 && B % 4 == 0 && A * a.nt[d] * nbx4 < (1LL << 31))
+    // 4-wide accesses need both base pointers aligned to the vector (a
+    // caller's view may start at any element)
+    const bool al4 = ((uintptr_t)src % (4 * sizeof(T))) == 0 && ((uintptr_t)dst % (4 * sizeof(T))) == 0;
+    if (B >= 64 && B % 4 == 0 && al4 && A * a.nt[d] * nbx4 < (1LL << 31))
       hipLaunchKernelGGL(resize_rows4<T>, dim3((unsigned)(A * a.nt[d] * nbx4)), dim3(RS_BLOCK), 0, st, src, dst,
                          (int)nbx4, B, a.ns[d], a.nt[d], ntap, tw, ti, acc);
     else if (B >= 64 && A * a.nt[d] * nbx < (1LL << 31))

@@ -192,3 +192,17 @@ def test_full_size_adjoint_and_prolongation():
     p1 = RZ.prolongation(y, torch.zeros_like(x))
     p2 = RZ.c_prolongation(y)
     assert rel(p1, p2.double().cpu().numpy()) <= 1e-5
+
+
+def test_resize_view_at_odd_offset():
+    """A contiguous input that starts one element into its storage (not
+    aligned to 16 bytes) gives the same result as an aligned copy."""
+    from nitorch_amd import resize as RZ
+    shape = (1, 2, 12, 16, 72)
+    n = int(np.prod(shape))
+    big = torch.randn(n + 1, device=DEV)
+    x = big[1:].view(shape)
+    assert x.is_contiguous() and x.data_ptr() % 16 != 0
+    a = RZ.resize(x, None, [0.5], ["dct2"], [3], ["edge"])
+    b = RZ.resize(x.clone(), None, [0.5], ["dct2"], [3], ["edge"])
+    assert torch.equal(a, b)
