@@ -262,6 +262,125 @@ __global__ __launch_bounds__(UN_BLOCK) void conv3_fwd_lds(const bf16 *__restrict
 }
 
 // ------------------------------------------------------------------------
+// conv3_fwd_lds for wide inputs (CI a multiple of 32, 64 and 96 here): the
+// K order is tap-major with x outermost (tap = 9 dx + 3 dy + dz), so the 9 CI
+// K values of one dx are whole chunks and need only one x-plane of the box.
+// The work-group stages the three planes one after the other, each
+// (UN_TY + 2) x (UN_TZ + 2) voxels, the next one's loads in flight while
+// the current one is consumed; tiles, fragments and accumulation order are
+// those of conv3_fwd_lds / conv3_fwd (identical results).
+// ------------------------------------------------------------------------
+template <int CI>
+struct PlaneBytes {
+  static constexpr int box = (UN_TY + 2) * (UN_TZ + 2) * CI * 2;
+  static constexpr int value = box + 9 * CI / 8 * 4;    // + the offset table of one plane's K slices
+};
+
+template <int CI, int CO, int COS, int ACT, bool OUTF32>
+__global__ __launch_bounds__(UN_BLOCK) void conv3_fwd_plane(const bf16 *__restrict__ X, const bf16 *__restrict__ Wk,
+                                                            const float *__restrict__ bias, void *__restrict__ Yv,
+                                                            int N, int D, int H, int W) {
+  extern __shared__ __align__(16) unsigned char un_smem[];
+  bf16 *box = (bf16 *)un_smem;
+  int *tab = (int *)(un_smem + PlaneBytes<CI>::box);
+  constexpr int KP = KPad<CI>::value, NKP = 9 * CI / 32;    // K chunks per plane
+  constexpr int CB = CO / 16;
+  constexpr int BY = UN_TY + 2, BZ = UN_TZ + 2, V8 = CI / 8;
+  constexpr int RC = BZ * V8, NCH = BY * RC;
+  constexpr int NIT = (NCH + UN_BLOCK - 1) / UN_BLOCK;
+  static_assert(CI % 32 == 0 && KP == 27 * CI && CO % 16 == 0 && COS <= CO, "channel counts");
+  const int YT = (H + UN_TY - 1) / UN_TY, ZT = (W + UN_TZ - 1) / UN_TZ;
+  int64_t t = un_block();
+  const int zt = (int)(t % ZT); t /= ZT;
+  const int yt = (int)(t % YT); t /= YT;
+  const int x = (int)(t % D);
+  const int n = (int)(t / D);
+  if (n >= N) return;    // uniform over the work-group
+  const int y0 = yt * UN_TY - 1, z0 = zt * UN_TZ - 1;
+
+  // K slice of one plane (tap 3 dy + dz, channel ci) -> plane offset from (-1, -1)
+  for (int i = threadIdx.x; i < 9 * CI / 8; i += UN_BLOCK) {
+    const int tap = i * 8 / CI, ci = i * 8 - tap * CI;
+    tab[i] = (tap / 3 * BZ + tap % 3) * CI + ci;
+  }
+  uint4 v[NIT];
+  auto fetch = [&](int dx) {
+    const int xx = x - 1 + dx;
+    const bool xok = xx >= 0 && xx < D;
+#pragma unroll
+    for (int u = 0; u < NIT; ++u) {
+      const int i = u * UN_BLOCK + threadIdx.x;
+      const int by = i / RC, j = i - by * RC;
+      const int yy = y0 + by, zz = z0 + j / V8;
+      v[u] = (uint4){0u, 0u, 0u, 0u};
+      if (xok && i < NCH && yy >= 0 && yy < H && zz >= 0 && zz < W)
+        v[u] = *(const uint4 *)(X + ((((int64_t)n * D + xx) * H + yy) * W + z0) * CI + j * 8);
+    }
+  };
+  fetch(0);
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = lane & 15, ks = lane >> 4;
+  const bf16 *abase = box + ((2 * wave) * BZ + row) * CI;
+  const int *tl = tab + ks;
+  const bf16 *wbase = Wk + (size_t)row * KP + ks * 8;
+  f32x4 acc[UN_MB][CB];
+#pragma unroll
+  for (int m = 0; m < UN_MB; ++m)
+#pragma unroll
+    for (int c = 0; c < CB; ++c) acc[m][c] = (f32x4){0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll 1
+  for (int dx = 0; dx < 3; ++dx) {
+    if (dx) __syncthreads();    // the previous plane is consumed
+#pragma unroll
+    for (int u = 0; u < NIT; ++u) {
+      const int i = u * UN_BLOCK + threadIdx.x;
+      if (i < NCH) *(uint4 *)(box + i * 8) = v[u];
+    }
+    __syncthreads();
+    if (dx < 2) fetch(dx + 1);
+    const bf16 *wp = wbase + dx * NKP * 32;
+#pragma unroll
+    for (int kp = 0; kp < NKP; ++kp) {
+      const bf16 *ap = abase + tl[kp * 4];
+      bf16x8 b[CB];
+#pragma unroll
+      for (int c = 0; c < CB; ++c) b[c] = *(const bf16x8 *)(wp + (size_t)c * 16 * KP + kp * 32);
+#pragma unroll
+      for (int m = 0; m < UN_MB; ++m) {
+        const bf16x8 a = *(const bf16x8 *)(ap + ((m >> 1) * BZ + (m & 1) * 16) * CI);
+#pragma unroll
+        for (int c = 0; c < CB; ++c)
+          acc[m][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[c], acc[m][c], 0, 0, 0);
+      }
+    }
+  }
+
+  const int yb = yt * UN_TY + 2 * wave, zb = zt * UN_TZ + ks * 4;
+  const size_t obase = ((((size_t)n * D + x) * H + yb) * W + zb) * COS;
+#pragma unroll
+  for (int c = 0; c < CB; ++c) {
+    const int co = c * 16 + row;
+    if (co >= COS) continue;
+    const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int m = 0; m < UN_MB; ++m) {
+      if (yb + (m >> 1) >= H) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (zb + (m & 1) * 16 + r >= W) continue;
+        float val = acc[m][c][r] + bb;
+        if (ACT == 1) val = val > 0.f ? val : UN_SLOPE * val;
+        const size_t o = obase + ((size_t)(m >> 1) * W + (m & 1) * 16 + r) * COS + co;
+        if (OUTF32) ((float *)Yv)[o] = val;
+        else ((bf16 *)Yv)[o] = (bf16)val;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
 // dW[co][tap * CI + ci] += sum_p dY[p][co] * X[p + off(tap)][ci]  (fp32, [CO][KP])
 // One wave: one 16-channel block of CO, UN_NT column blocks of (tap, ci),
 // `rows` consecutive (n, x, y) rows, all z.
@@ -510,21 +629,23 @@ static inline unsigned nblk(int64_t n) { return (unsigned)((n + UN_BLOCK - 1) / 
   X_(16, 16) X_(16, 32) X_(32, 32) X_(32, 64) X_(64, 64) X_(96, 32) X_(48, 16) \
   X_(32, 16) X_(64, 32) X_(32, 96) X_(16, 48)
 
-// Raise the dynamic-LDS limit of one conv3_fwd_lds instantiation to the
-// device's maximum (once per device); false if the box does not fit.
-template <int CI, int CO, int COS, int ACT, bool OUTF32>
+// Raise the dynamic-LDS limit of one staged-convolution instantiation to
+// `bytes` (once per device); false if that does not fit.
+template <int CI, int CO, int COS, int ACT, bool OUTF32, bool PLANE>
 static bool lds_ready() {
-  constexpr int bytes = BoxBytes<CI>::value;
+  constexpr int bytes = PLANE ? PlaneBytes<CI>::value : BoxBytes<CI>::value;
   static int state[64] = {0};    // 0 unknown, 1 usable, 2 not
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return false;
   if (state[dev] == 0) {
     int v = 0;
     bool ok = bytes <= 65536;
+    const void *fn;
+    if constexpr (PLANE) fn = (const void *)conv3_fwd_plane<CI, CO, COS, ACT, OUTF32>;
+    else fn = (const void *)conv3_fwd_lds<CI, CO, COS, ACT, OUTF32>;
     if (!ok && hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
         v >= bytes)
-      ok = hipFuncSetAttribute((const void *)conv3_fwd_lds<CI, CO, COS, ACT, OUTF32>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess;
+      ok = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess;
     (void)hipGetLastError();
     state[dev] = ok ? 1 : 2;
   }
@@ -534,12 +655,19 @@ static bool lds_ready() {
 template <int CI, int CO, int COS, int ACT, bool OUTF32>
 static void launch_fwd1(const bf16 *x, const bf16 *w, const float *b, void *y, int N, int D, int H, int W,
                         hipStream_t st) {
-  // the staged kernel while its box fits a CU comfortably (CI <= 48: <= 99 KB of LDS)
+  // the staged kernel while its box fits a CU comfortably (CI <= 48: <= 99 KB
+  // of LDS); wider inputs stream the box plane by plane (<= 66 KB at CI = 96)
+  const int64_t tiles = (int64_t)N * D * ((H + UN_TY - 1) / UN_TY) * ((W + UN_TZ - 1) / UN_TZ);
   if constexpr (CI <= 48) {
-    const int64_t tiles = (int64_t)N * D * ((H + UN_TY - 1) / UN_TY) * ((W + UN_TZ - 1) / UN_TZ);
-    if (tiles < (int64_t(1) << 31) && lds_ready<CI, CO, COS, ACT, OUTF32>()) {
+    if (tiles < (int64_t(1) << 31) && lds_ready<CI, CO, COS, ACT, OUTF32, false>()) {
       hipLaunchKernelGGL((conv3_fwd_lds<CI, CO, COS, ACT, OUTF32>), dim3((unsigned)tiles), dim3(UN_BLOCK),
                          BoxBytes<CI>::value, st, x, w, b, y, N, D, H, W);
+      return;
+    }
+  } else if constexpr (CI % 32 == 0) {
+    if (tiles < (int64_t(1) << 31) && lds_ready<CI, CO, COS, ACT, OUTF32, true>()) {
+      hipLaunchKernelGGL((conv3_fwd_plane<CI, CO, COS, ACT, OUTF32>), dim3((unsigned)tiles), dim3(UN_BLOCK),
+                         PlaneBytes<CI>::value, st, x, w, b, y, N, D, H, W);
       return;
     }
   }
