@@ -143,132 +143,26 @@ __global__ __launch_bounds__(UN_BLOCK) void conv3_fwd(const bf16 *__restrict__ X
 // The same convolution with the input staged in LDS.  A work-group owns a
 // tile of one (n, x) plane: UN_TY = 8 rows of y by UN_TZ = 32 positions of
 // z; wave w computes rows 2w and 2w + 1, each as two 16-position row blocks
-// (m = 2 * (row - 2w) + z block).  The box of input voxels the tile's taps
-// reach, 3 x (UN_TY + 2) x (UN_TZ + 2) voxels of CI channels, is loaded once
-// (zero outside the volume, which is the convolution's zero padding) and the
-// 27 taps read it from LDS: each input voxel leaves L2 about 4 times per
-// layer instead of 27.  Fragments, K order and accumulation order are those
-// of conv3_fwd, so the two kernels give identical results.
+// (m = 2 * (row - 2w) + z block).  The input voxels the tile's taps reach,
+// x - 1 .. x + 1 by (UN_TY + 2) x (UN_TZ + 2), are staged in LDS (zero
+// outside the volume, which is the convolution's zero padding) and the 27
+// taps read them from LDS: each input voxel leaves L2 about 4 times per
+// layer instead of 27.  A K-slice -> LDS-offset table makes each 32-deep K
+// chunk one table read and one add; the four A fragments of a chunk are LDS
+// reads at immediate offsets.  Fragments, K order and accumulation order
+// are those of conv3_fwd, so the kernels give identical results.
 // ------------------------------------------------------------------------
 #define UN_TY 8
 #define UN_TZ 32
 
-template <int CI>
-struct BoxBytes {
-  static constexpr int box = 3 * (UN_TY + 2) * (UN_TZ + 2) * CI * 2;
-  static constexpr int value = box + KPad<CI>::value / 8 * 4;    // + the K-slice offset table
-};
-
-template <int CI, int CO, int COS, int ACT, bool OUTF32>
-__global__ __launch_bounds__(UN_BLOCK) void conv3_fwd_lds(const bf16 *__restrict__ X, const bf16 *__restrict__ Wk,
-                                                          const float *__restrict__ bias, void *__restrict__ Yv,
-                                                          int N, int D, int H, int W) {
-  extern __shared__ __align__(16) unsigned char un_smem[];
-  bf16 *box = (bf16 *)un_smem;
-  int *tab = (int *)(un_smem + BoxBytes<CI>::box);
-  constexpr int KP = KPad<CI>::value, NKC = KP / 32;
-  constexpr int CB = CO / 16;
-  constexpr int BY = UN_TY + 2, BZ = UN_TZ + 2, V8 = CI / 8;
-  constexpr int RC = BZ * V8;            // 16-byte chunks in one box row (BZ voxels)
-  constexpr int NCH = 3 * BY * RC;
-  // 27 CI is a multiple of 16: the last K chunk is either full or ends in 16 zero-weight slots (ks >= 2)
-  constexpr bool PAD = KP != 27 * CI;
-  static_assert(CI % 16 == 0 && CO % 16 == 0 && COS <= CO, "channel counts");
-  static_assert(!PAD || KP - 27 * CI == 16, "K padding is half a chunk");
-  const int YT = (H + UN_TY - 1) / UN_TY, ZT = (W + UN_TZ - 1) / UN_TZ;
-  int64_t t = un_block();
-  const int zt = (int)(t % ZT); t /= ZT;
-  const int yt = (int)(t % YT); t /= YT;
-  const int x = (int)(t % D);
-  const int n = (int)(t / D);
-  if (n >= N) return;    // uniform over the work-group
-  const int x0 = x - 1, y0 = yt * UN_TY - 1, z0 = zt * UN_TZ - 1;
-
-  // K slice k / 8 -> box offset of its 8 channels from the tap (-1, -1, -1)
-  // voxel of an output position (0 for the zero-weight padding)
-  for (int i = threadIdx.x; i < KP / 8; i += UN_BLOCK) {
-    const int tap = i * 8 / CI, ci = i * 8 - tap * CI;
-    tab[i] = tap < 27 ? ((tap / 9 * BY + (tap / 3) % 3) * BZ + tap % 3) * CI + ci : 0;
-  }
-  // box rows (bx, by): BZ voxels of CI channels, one contiguous span of X;
-  // all of a thread's loads are issued before its first LDS store
-  constexpr int NIT = (NCH + UN_BLOCK - 1) / UN_BLOCK;
-  uint4 v[NIT];
-#pragma unroll
-  for (int u = 0; u < NIT; ++u) {
-    const int i = u * UN_BLOCK + threadIdx.x;
-    const int rw = i / RC, j = i - rw * RC;
-    const int bx = rw / BY, by = rw - bx * BY;
-    const int xx = x0 + bx, yy = y0 + by, zz = z0 + j / V8;
-    v[u] = (uint4){0u, 0u, 0u, 0u};
-    if (i < NCH && xx >= 0 && xx < D && yy >= 0 && yy < H && zz >= 0 && zz < W)
-      v[u] = *(const uint4 *)(X + ((((int64_t)n * D + xx) * H + yy) * W + z0) * CI + j * 8);
-  }
-#pragma unroll
-  for (int u = 0; u < NIT; ++u) {
-    const int i = u * UN_BLOCK + threadIdx.x;
-    if (i < NCH) *(uint4 *)(box + i * 8) = v[u];
-  }
-  __syncthreads();
-
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int row = lane & 15, ks = lane >> 4;
-  // this lane's A row for m = 0 (box row 2 wave, z = row) and B column
-  const bf16 *abase = box + ((2 * wave) * BZ + row) * CI;
-  const int *tl = tab + ks;
-  const bf16 *wbase = Wk + (size_t)row * KP + ks * 8;
-  f32x4 acc[UN_MB][CB];
-#pragma unroll
-  for (int m = 0; m < UN_MB; ++m)
-#pragma unroll
-    for (int c = 0; c < CB; ++c) acc[m][c] = (f32x4){0.f, 0.f, 0.f, 0.f};
-
-#pragma unroll
-  for (int kc = 0; kc < NKC; ++kc) {
-    const bf16 *ap = abase + tl[kc * 4];
-    bf16x8 b[CB];
-#pragma unroll
-    for (int c = 0; c < CB; ++c) b[c] = *(const bf16x8 *)(wbase + (size_t)c * 16 * KP + kc * 32);
-#pragma unroll
-    for (int m = 0; m < UN_MB; ++m) {
-      bf16x8 a = *(const bf16x8 *)(ap + ((m >> 1) * BZ + (m & 1) * 16) * CI);
-      if (PAD && kc == NKC - 1 && ks >= 2) a = zero8();
-#pragma unroll
-      for (int c = 0; c < CB; ++c) acc[m][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[c], acc[m][c], 0, 0, 0);
-    }
-  }
-
-  const int yb = yt * UN_TY + 2 * wave, zb = zt * UN_TZ + ks * 4;
-  const size_t obase = ((((size_t)n * D + x) * H + yb) * W + zb) * COS;
-#pragma unroll
-  for (int c = 0; c < CB; ++c) {
-    const int co = c * 16 + row;
-    if (co >= COS) continue;
-    const float bb = bias ? bias[co] : 0.f;
-#pragma unroll
-    for (int m = 0; m < UN_MB; ++m) {
-      if (yb + (m >> 1) >= H) continue;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        if (zb + (m & 1) * 16 + r >= W) continue;
-        float val = acc[m][c][r] + bb;
-        if (ACT == 1) val = val > 0.f ? val : UN_SLOPE * val;
-        const size_t o = obase + ((size_t)(m >> 1) * W + (m & 1) * 16 + r) * COS + co;
-        if (OUTF32) ((float *)Yv)[o] = val;
-        else ((bf16 *)Yv)[o] = (bf16)val;
-      }
-    }
-  }
-}
-
 // ------------------------------------------------------------------------
-// conv3_fwd_lds for wide inputs (CI a multiple of 32, 64 and 96 here): the
+// Staged convolution for wide inputs (CI a multiple of 32, 64 and 96 here): the
 // K order is tap-major with x outermost (tap = 9 dx + 3 dy + dz), so the 9 CI
 // K values of one dx are whole chunks and need only one x-plane of the box.
 // The work-group stages the three planes one after the other, each
 // (UN_TY + 2) x (UN_TZ + 2) voxels, the next one's loads in flight while
 // the current one is consumed; tiles, fragments and accumulation order are
-// those of conv3_fwd_lds / conv3_fwd (identical results).
+// those of conv3_fwd (identical results).
 // ------------------------------------------------------------------------
 template <int CI>
 struct PlaneBytes {
@@ -356,6 +250,133 @@ __global__ __launch_bounds__(UN_BLOCK) void conv3_fwd_plane(const bf16 *__restri
       }
     }
   }
+
+  const int yb = yt * UN_TY + 2 * wave, zb = zt * UN_TZ + ks * 4;
+  const size_t obase = ((((size_t)n * D + x) * H + yb) * W + zb) * COS;
+#pragma unroll
+  for (int c = 0; c < CB; ++c) {
+    const int co = c * 16 + row;
+    if (co >= COS) continue;
+    const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int m = 0; m < UN_MB; ++m) {
+      if (yb + (m >> 1) >= H) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (zb + (m & 1) * 16 + r >= W) continue;
+        float val = acc[m][c][r] + bb;
+        if (ACT == 1) val = val > 0.f ? val : UN_SLOPE * val;
+        const size_t o = obase + ((size_t)(m >> 1) * W + (m & 1) * 16 + r) * COS + co;
+        if (OUTF32) ((float *)Yv)[o] = val;
+        else ((bf16 *)Yv)[o] = (bf16)val;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// Staged convolution for inputs of up to 48 channels, in two x-plane slots
+// (16 | CI): the K values of x-offsets 0 and 1 end at chunk 9 CI / 16, so
+// the work-group stages planes x - 1 and x into slots 0 and 1, consumes
+// those chunks, then stages plane x + 1 (its loads in flight meanwhile)
+// into slot 0 for the rest: two thirds of the whole box's LDS (65 KB at
+// CI = 48), so twice the work-groups per CU at CI = 48.
+// ------------------------------------------------------------------------
+template <int CI>
+struct SlotBytes {
+  static constexpr int plane = (UN_TY + 2) * (UN_TZ + 2) * CI;    // elements
+  static constexpr int value = 2 * plane * 2 + KPad<CI>::value / 8 * 4;
+};
+
+template <int CI, int CO, int COS, int ACT, bool OUTF32>
+__global__ __launch_bounds__(UN_BLOCK) void conv3_fwd_2slot(const bf16 *__restrict__ X, const bf16 *__restrict__ Wk,
+                                                            const float *__restrict__ bias, void *__restrict__ Yv,
+                                                            int N, int D, int H, int W) {
+  extern __shared__ __align__(16) unsigned char un_smem[];
+  constexpr int PE = SlotBytes<CI>::plane;
+  bf16 *box = (bf16 *)un_smem;
+  int *tab = (int *)(un_smem + 2 * PE * 2);
+  constexpr int KP = KPad<CI>::value, NKC = KP / 32, NKA = 9 * CI / 16;    // chunks of x-offsets 0, 1
+  constexpr int CB = CO / 16;
+  constexpr int BY = UN_TY + 2, BZ = UN_TZ + 2, V8 = CI / 8;
+  constexpr int RC = BZ * V8, NCH = BY * RC;
+  constexpr int NIT = (NCH + UN_BLOCK - 1) / UN_BLOCK;
+  constexpr bool PAD = KP != 27 * CI;
+  static_assert(CI % 16 == 0 && CO % 16 == 0 && COS <= CO, "channel counts");
+  static_assert(!PAD || KP - 27 * CI == 16, "K padding is half a chunk");
+  const int YT = (H + UN_TY - 1) / UN_TY, ZT = (W + UN_TZ - 1) / UN_TZ;
+  int64_t t = un_block();
+  const int zt = (int)(t % ZT); t /= ZT;
+  const int yt = (int)(t % YT); t /= YT;
+  const int x = (int)(t % D);
+  const int n = (int)(t / D);
+  if (n >= N) return;    // uniform over the work-group
+  const int y0 = yt * UN_TY - 1, z0 = zt * UN_TZ - 1;
+
+  // K slice -> slot (x-offset 0, 2: slot 0; 1: slot 1) and plane offset
+  for (int i = threadIdx.x; i < KP / 8; i += UN_BLOCK) {
+    const int tap = i * 8 / CI, ci = i * 8 - tap * CI;
+    tab[i] = tap < 27 ? (tap / 9 & 1) * PE + ((tap / 3) % 3 * BZ + tap % 3) * CI + ci : 0;
+  }
+  uint4 v[NIT];
+  auto fetch = [&](int dx) {
+    const int xx = x - 1 + dx;
+    const bool xok = xx >= 0 && xx < D;
+#pragma unroll
+    for (int u = 0; u < NIT; ++u) {
+      const int i = u * UN_BLOCK + threadIdx.x;
+      const int by = i / RC, j = i - by * RC;
+      const int yy = y0 + by, zz = z0 + j / V8;
+      v[u] = (uint4){0u, 0u, 0u, 0u};
+      if (xok && i < NCH && yy >= 0 && yy < H && zz >= 0 && zz < W)
+        v[u] = *(const uint4 *)(X + ((((int64_t)n * D + xx) * H + yy) * W + z0) * CI + j * 8);
+    }
+  };
+  auto store = [&](int slot) {
+#pragma unroll
+    for (int u = 0; u < NIT; ++u) {
+      const int i = u * UN_BLOCK + threadIdx.x;
+      if (i < NCH) *(uint4 *)(box + slot * PE + i * 8) = v[u];
+    }
+  };
+  fetch(0);
+  store(0);
+  fetch(1);
+  store(1);
+  __syncthreads();
+  fetch(2);
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = lane & 15, ks = lane >> 4;
+  const bf16 *abase = box + ((2 * wave) * BZ + row) * CI;
+  const int *tl = tab + ks;
+  const bf16 *wbase = Wk + (size_t)row * KP + ks * 8;
+  f32x4 acc[UN_MB][CB];
+#pragma unroll
+  for (int m = 0; m < UN_MB; ++m)
+#pragma unroll
+    for (int c = 0; c < CB; ++c) acc[m][c] = (f32x4){0.f, 0.f, 0.f, 0.f};
+
+  auto chunk = [&](int kc) {
+    const bf16 *ap = abase + tl[kc * 4];
+    bf16x8 b[CB];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) b[c] = *(const bf16x8 *)(wbase + (size_t)c * 16 * KP + kc * 32);
+#pragma unroll
+    for (int m = 0; m < UN_MB; ++m) {
+      bf16x8 a = *(const bf16x8 *)(ap + ((m >> 1) * BZ + (m & 1) * 16) * CI);
+      if (PAD && kc == NKC - 1 && ks >= 2) a = zero8();
+#pragma unroll
+      for (int c = 0; c < CB; ++c) acc[m][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[c], acc[m][c], 0, 0, 0);
+    }
+  };
+#pragma unroll
+  for (int kc = 0; kc < NKA; ++kc) chunk(kc);
+  __syncthreads();    // slot 0 (plane x - 1) is consumed
+  store(0);
+  __syncthreads();
+#pragma unroll
+  for (int kc = NKA; kc < NKC; ++kc) chunk(kc);
 
   const int yb = yt * UN_TY + 2 * wave, zb = zt * UN_TZ + ks * 4;
   const size_t obase = ((((size_t)n * D + x) * H + yb) * W + zb) * COS;
@@ -631,9 +652,9 @@ static inline unsigned nblk(int64_t n) { return (unsigned)((n + UN_BLOCK - 1) / 
 
 // Raise the dynamic-LDS limit of one staged-convolution instantiation to
 // `bytes` (once per device); false if that does not fit.
-template <int CI, int CO, int COS, int ACT, bool OUTF32, bool PLANE>
+template <int CI, int CO, int COS, int ACT, bool OUTF32, bool PLANE>    // conv3_fwd_plane, else conv3_fwd_2slot
 static bool lds_ready() {
-  constexpr int bytes = PLANE ? PlaneBytes<CI>::value : BoxBytes<CI>::value;
+  constexpr int bytes = PLANE ? PlaneBytes<CI>::value : SlotBytes<CI>::value;
   static int state[64] = {0};    // 0 unknown, 1 usable, 2 not
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return false;
@@ -642,7 +663,7 @@ static bool lds_ready() {
     bool ok = bytes <= 65536;
     const void *fn;
     if constexpr (PLANE) fn = (const void *)conv3_fwd_plane<CI, CO, COS, ACT, OUTF32>;
-    else fn = (const void *)conv3_fwd_lds<CI, CO, COS, ACT, OUTF32>;
+    else fn = (const void *)conv3_fwd_2slot<CI, CO, COS, ACT, OUTF32>;
     if (!ok && hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
         v >= bytes)
       ok = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess;
@@ -655,13 +676,13 @@ static bool lds_ready() {
 template <int CI, int CO, int COS, int ACT, bool OUTF32>
 static void launch_fwd1(const bf16 *x, const bf16 *w, const float *b, void *y, int N, int D, int H, int W,
                         hipStream_t st) {
-  // the staged kernel while its box fits a CU comfortably (CI <= 48: <= 99 KB
-  // of LDS); wider inputs stream the box plane by plane (<= 66 KB at CI = 96)
+  // staged kernels: two plane slots up to 48 input channels (<= 66 KB of
+  // LDS), one plane at a time for 64 and 96 (<= 66 KB at CI = 96)
   const int64_t tiles = (int64_t)N * D * ((H + UN_TY - 1) / UN_TY) * ((W + UN_TZ - 1) / UN_TZ);
   if constexpr (CI <= 48) {
     if (tiles < (int64_t(1) << 31) && lds_ready<CI, CO, COS, ACT, OUTF32, false>()) {
-      hipLaunchKernelGGL((conv3_fwd_lds<CI, CO, COS, ACT, OUTF32>), dim3((unsigned)tiles), dim3(UN_BLOCK),
-                         BoxBytes<CI>::value, st, x, w, b, y, N, D, H, W);
+      hipLaunchKernelGGL((conv3_fwd_2slot<CI, CO, COS, ACT, OUTF32>), dim3((unsigned)tiles), dim3(UN_BLOCK),
+                         SlotBytes<CI>::value, st, x, w, b, y, N, D, H, W);
       return;
     }
   } else if constexpr (CI % 32 == 0) {
