@@ -293,6 +293,22 @@ int ni_unet_pack_input(int64_t N, int64_t V, const float *g, const float *h, int
 int ni_unet_unpack_output(int64_t N, int64_t V, const float *y, float *x, float scale, int32_t accumulate,
                           void *stream);
 
+/* ---- SVF exponentiation backward: fields of 3 x 3 matrices ----
+ * Matrix fields are addressed by element strides {per voxel, per row, per
+ * column}, vector fields by {per voxel, per component}, over `nvox`
+ * voxels; outputs are dense.  These replace the torch products of the
+ * reference's exp_backward / _composition_jac (nitorch/spatial/_svf.py:
+ * 271-284, linalg.matvec and jac.det() per squaring, and :480,
+ * torch.matmul(jac_left, jac)). */
+
+/* out [nvox, 3, 3] = a @ b */
+int ni_svf_mat3_mul(int32_t dtype, int64_t nvox, const void *a, const int64_t *a_stride, const void *b,
+                    const int64_t *b_stride, void *out, void *stream);
+
+/* out [nvox, 3] = g0 + det(jac) * jac^T @ p  (one squaring's push-forward) */
+int ni_svf_grad_step(int32_t dtype, int64_t nvox, const void *jac, const int64_t *jac_stride, const void *p,
+                     const int64_t *p_stride, const void *g0, const int64_t *g0_stride, void *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

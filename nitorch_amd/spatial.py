@@ -381,35 +381,35 @@ def _diff_central(x, dims, bound="dct2", voxel_size=1):
     """First-order central differences of x along `dims`, stacked last
     (nitorch/spatial/_finite_differences.py:154-184, `diff(side='c')`):
     (x[i+1] - x[i-1]) / (2 vs) inside, the bound's rule at both ends, and a
-    copy of x along an axis of length 1."""
+    copy of x along an axis of length 1.  Each axis's differences are
+    computed into a dense tensor of x's layout (no intermediate copies)."""
     vs = list(voxel_size) if isinstance(voxel_size, (list, tuple)) else [voxel_size]
     vs = (vs + vs[-1:] * len(dims))[:len(dims)]
+    if bound not in ("dct2", "replicate", "dct1", "dst2", "dst1", "zero", "dft"):
+        raise ValueError("Unknown boundary type {}.".format(bound))
     out = []
     for d, v in zip(dims, vs):
         xm = x.movedim(d, 0)
-        n = xm.shape[0]
-        if n == 1:
-            out.append(xm.clone().movedim(0, d))
+        if xm.shape[0] == 1:
+            out.append(x.clone())
             continue
         df = torch.empty_like(xm)
-        df[1:-1] = xm[2:] - xm[:-2]
+        torch.sub(xm[2:], xm[:-2], out=df[1:-1])
         if bound in ("dct2", "replicate"):
-            df[0] = xm[1] - xm[0]
-            df[-1] = xm[-1] - xm[-2]
+            torch.sub(xm[1], xm[0], out=df[0])
+            torch.sub(xm[-1], xm[-2], out=df[-1])
         elif bound == "dct1":
-            df[0] = 0
-            df[-1] = 0
+            df[0].zero_()
+            df[-1].zero_()
         elif bound == "dst2":
-            df[0] = xm[1] + xm[0]
-            df[-1] = -(xm[-1] + xm[-2])
+            torch.add(xm[1], xm[0], out=df[0])
+            torch.add(xm[-1], xm[-2], out=df[-1]).neg_()
         elif bound in ("dst1", "zero"):
-            df[0] = xm[1]
-            df[-1] = -xm[-2]
-        elif bound == "dft":
-            df[0] = xm[1] - xm[-1]
-            df[-1] = xm[0] - xm[-2]
-        else:
-            raise ValueError("Unknown boundary type {}.".format(bound))
+            df[0].copy_(xm[1])
+            torch.neg(xm[-2], out=df[-1])
+        else:    # dft
+            torch.sub(xm[1], xm[-1], out=df[0])
+            torch.sub(xm[0], xm[-2], out=df[-1])
         out.append(df.div_(2 * v if v != 1 else 2.).movedim(0, d))
     return torch.stack(out, -1)
 

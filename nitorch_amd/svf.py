@@ -13,11 +13,42 @@ the squarings (`anagrad=True`).
 
 Fields are channel-last like the reference: ([batch], *spatial, dim);
 Jacobians (..., dim, dim) with jac[..., i, j] = d psi_i / d x_j; symmetric
-Hessians stored sparse as (h00, h11, h22, h01, h02, h12).
+Hessians stored sparse as (h00, h11, h22, h01, h02, h12).  3-D fields of
+3 x 3 matrices on the GPU are multiplied, and pushed forward through a
+squaring, by libnitorch_amd.so (`ni_svf_mat3_mul`, `ni_svf_grad_step`),
+reading transposed and channel-first views in place.
 """
 import torch
 
+from . import _lib as L
 from . import spatial as SP
+
+L.register_signatures({
+    "ni_svf_mat3_mul": [L._I32, L._I64, L._P, L._PI64, L._P, L._PI64, L._P, L._P],
+    "ni_svf_grad_step": [L._I32, L._I64, L._P, L._PI64, L._P, L._PI64, L._P, L._PI64, L._P, L._P],
+})
+
+
+def _vox_stride(t, trailing):
+    """Element stride of the voxel index when all but the `trailing` dims of
+    t collapse into one (None if they do not)."""
+    lead = [(n, st) for n, st in zip(t.shape[:t.dim() - trailing], t.stride()[:t.dim() - trailing]) if n != 1]
+    if not lead:
+        return 0
+    step = lead[-1][1]
+    span = step * lead[-1][0]
+    for n, st in reversed(lead[:-1]):
+        if st != span:
+            return None
+        span *= n
+    return step
+
+
+def _native3(*ts):
+    """The GPU kernels apply: 3-D float32 / float64 CUDA fields of one shape."""
+    t0 = ts[0]
+    return (t0.is_cuda and t0.dtype in (torch.float32, torch.float64) and t0.shape[-1] == 3
+            and all(t.device == t0.device and t.dtype == t0.dtype for t in ts))
 
 
 def _pull_vel(vel, grid, interpolation="linear", bound="dft", extrapolate=True):
@@ -84,14 +115,59 @@ def _composition_jac(jac, rhs, lhs=None, type="grid", identity=None, **kwargs):
     return _matmul(jac_left, jac)
 
 
-# Products of fields of small matrices as broadcast multiply-and-sum (one
-# elementwise kernel each) rather than batched GEMMs of millions of 3 x 3s.
+# Products of fields of small matrices as sums of d rank-one terms, each
+# one elementwise kernel over the output (in-place multiply-adds), rather
+# than batched GEMMs of millions of 3 x 3s or a d-times wider temporary
+# reduced over a strided axis.
 def _matvec(mat, vec):
-    return (mat * vec[..., None, :]).sum(-1)
+    out = mat[..., :, 0] * vec[..., 0:1]
+    for k in range(1, mat.shape[-1]):
+        out.addcmul_(mat[..., :, k], vec[..., k:k + 1])
+    return out
+
+
+def _mat3_into(a, b, out):
+    """out = a @ b on the GPU; leading dims that do not collapse into one
+    voxel stride are split.  False if a view cannot be addressed."""
+    sa, sb = _vox_stride(a, 2), _vox_stride(b, 2)
+    if sa is None or sb is None:
+        return a.dim() > 3 and all(_mat3_into(a[i], b[i], out[i]) for i in range(a.shape[0]))
+    L.check(L.lib().ni_svf_mat3_mul(L.dtype_code(a.dtype), out.numel() // 9, L.ptr(a),
+                                    L.i64arr([sa, a.stride(-2), a.stride(-1)]), L.ptr(b),
+                                    L.i64arr([sb, b.stride(-2), b.stride(-1)]), L.ptr(out),
+                                    L.stream_of(a)), "svf_mat3_mul")
+    return True
 
 
 def _matmul(a, b):
-    return (a[..., :, :, None] * b[..., None, :, :]).sum(-2)
+    if _native3(a, b) and a.shape == b.shape and a.shape[-2] == 3:
+        out = torch.empty(a.shape, dtype=a.dtype, device=a.device)
+        if _mat3_into(a, b, out):
+            return out
+    out = a[..., :, 0:1] * b[..., 0:1, :]
+    for k in range(1, a.shape[-1]):
+        out.addcmul_(a[..., :, k:k + 1], b[..., k:k + 1, :])
+    return out
+
+
+def _step_into(jac, p, g0, out):
+    sj, sp, sg = _vox_stride(jac, 2), _vox_stride(p, 1), _vox_stride(g0, 1)
+    if sj is None or sp is None or sg is None:
+        return p.dim() > 2 and all(_step_into(jac[i], p[i], g0[i], out[i]) for i in range(p.shape[0]))
+    L.check(L.lib().ni_svf_grad_step(L.dtype_code(jac.dtype), out.numel() // 3, L.ptr(jac),
+                                     L.i64arr([sj, jac.stride(-2), jac.stride(-1)]), L.ptr(p),
+                                     L.i64arr([sp, p.stride(-1)]), L.ptr(g0), L.i64arr([sg, g0.stride(-1)]),
+                                     L.ptr(out), L.stream_of(g0)), "svf_grad_step")
+    return True
+
+
+def _grad_step(jac, p, g0):
+    """g0 + det(jac) jac^T p (one squaring's push-forward of the gradient,
+    _svf.py:272-279), fused for 3-D GPU fields; None if not applicable."""
+    if not (_native3(jac, p, g0) and jac.shape[:-1] == p.shape == g0.shape and jac.shape[-2] == 3):
+        return None
+    out = torch.empty(g0.shape, dtype=g0.dtype, device=g0.device)
+    return out if _step_into(jac, p, g0, out) else None
 
 
 def _det(m):
@@ -189,11 +265,17 @@ def exp_backward(vel, *gradhess, inverse=False, steps=8, interpolation="linear",
     grad = _matvec(ijac, grad)
     del ijac
     for _ in range(steps):
+        pulled = _pull_vel(grad, ident + vel, **opt)
+        step = _grad_step(jac, pulled, grad) if hess is None else None
+        if step is not None:
+            grad = step
+            jac = _composition_jac(jac, vel, type="disp", identity=ident, **opt)
+            vel += _pull_vel(vel, ident + vel, **opt)
+            continue
         det = _det(jac)
         jac = jac.transpose(-1, -2)
         grad0 = grad
-        grad = _pull_vel(grad, ident + vel, **opt)
-        grad = _matvec(jac, grad)
+        grad = _matvec(jac, pulled)
         grad *= det[..., None]
         grad += grad0
         if hess is not None:

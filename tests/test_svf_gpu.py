@@ -136,3 +136,36 @@ def test_exp_backward_full_size_closed_form():
     out = svf.exp_backward(v, g, steps=s, rotate_grad=True)
     ref = sum(torch.roll(g, shifts=tuple(m * c for c in u), dims=(1, 2, 3)) for m in range(2 ** s)) / 2 ** s
     assert (out - ref).abs().max().item() <= 1e-5 * g.abs().max().item()
+
+
+@pytest.mark.parametrize("dt,tol", [(torch.float64, 1e-12), (torch.float32, 2e-5)])
+def test_mat3_kernels_vs_torch(dt, tol):
+    """ni_svf_mat3_mul / ni_svf_grad_step against float64 torch products, on
+    dense, transposed, channel-first (entries as channels) and stride-0
+    (expanded identity) views."""
+    from nitorch_amd import svf
+    g = torch.Generator(device="cpu").manual_seed(5)
+    shp = (2, 5, 6, 7)
+    a = torch.randn(shp + (3, 3), generator=g, dtype=torch.float64)
+    b = torch.randn(shp + (3, 3), generator=g, dtype=torch.float64)
+    p = torch.randn(shp + (3,), generator=g, dtype=torch.float64)
+    g0 = torch.randn(shp + (3,), generator=g, dtype=torch.float64)
+    A, B, P, G0 = (t.to(DEV, dt) for t in (a, b, p, g0))
+
+    def rel(x, ref):
+        return float((x.double().cpu() - ref).abs().max() / ref.abs().max())
+
+    def chan_first(t, k):    # same values, entries stored as leading channels
+        return t.reshape(shp + (k,)).movedim(-1, 1).contiguous().movedim(1, -1)
+
+    assert rel(svf._matmul(A, B), a @ b) <= tol
+    assert rel(svf._matmul(A.transpose(-1, -2), B), a.transpose(-1, -2) @ b) <= tol
+    Bc = chan_first(B, 9).reshape(shp + (3, 3))
+    assert Bc.stride(-1) != 1 and rel(svf._matmul(A, Bc), a @ b) <= tol
+    eye = torch.eye(3, dtype=dt, device=DEV).expand(shp + (3, 3))
+    assert torch.equal(svf._matmul(A, eye), A)
+    ref = g0 + torch.linalg.det(a)[..., None] * (a.transpose(-1, -2) @ p[..., None])[..., 0]
+    out = svf._grad_step(A, chan_first(P, 3), G0)
+    assert out is not None and rel(out, ref) <= tol
+    # not applicable: mismatched shapes fall back to the caller's torch path
+    assert svf._grad_step(A, P[:1], G0) is None
