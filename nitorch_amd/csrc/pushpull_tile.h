@@ -79,7 +79,12 @@ struct TileBox {
   static constexpr int DYZP = DYZ + ((48 - DYZ) % 64 + 64) % 64;
   static constexpr int VOL = DX * DYZP;
   static constexpr int NPP = (DYZ + NTH - 1) / NTH;     // y-z plane positions per thread
-  static constexpr int MAX_NCC = 2;                    // channels staged at once
+  // channels staged at once: 2 for the scatters (a 52 KB box at order 3)
+  // and for pull / grid gradient, 4 for grid_grad (its 3 accumulators per
+  // channel amortise the taps' weights and derivatives over more channels:
+  // 4x8x192^3, order 3, 3.13 -> 2.67 ms, while pull itself is 1.57 ms at 2
+  // and 1.75 ms at 4 channels)
+  static constexpr int MAX_NCC = SCAT ? 2 : 4;
   // waves per SIMD asked of the register allocator for the 512-thread
   // scatter: 6 (<= 85 VGPRs) lets three 52 KB boxes share a CU's LDS
   static constexpr int MINW = BIG ? 6 : 1;
@@ -898,7 +903,8 @@ static void launch_tile(const PPArgs<float> &a, hipStream_t st) {
     if (a.C == 1)
       hipLaunchKernelGGL((tile_kernel<ORD, MODE, 1>), g, b, 0, st, a, ntx, nty, ntz, nb, fb);
     else
-      hipLaunchKernelGGL((tile_kernel<ORD, MODE, B::MAX_NCC>), g, b, 0, st, a, ntx, nty, ntz, nb, fb);
+      hipLaunchKernelGGL((tile_kernel<ORD, MODE, MODE == TM_SGRAD ? B::MAX_NCC : 2>), g, b, 0, st, a, ntx, nty, ntz,
+                         nb, fb);
   }
   if (fb) {
     const unsigned gf = (unsigned)(nb < 2048 ? nb : 2048);
