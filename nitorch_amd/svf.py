@@ -226,9 +226,10 @@ def exp_forward(vel, inverse=False, steps=8, interpolation="linear", bound="dft"
     iadd = (lambda x, y: x.add(y)) if (not _anagrad and vel.requires_grad) else (lambda x, y: x.add_(y))
     vel = vel / (2 ** steps)
     for _ in range(steps):
+        grid = ident + vel
         if jacobian:
-            jac = _composition_jac(jac, vel, type="displacement", identity=ident)
-        vel = iadd(vel, _pull_vel(vel, ident + vel, interpolation=interpolation, bound=bound))
+            jac = _composition_jac(jac, grid, lhs=vel, type=("grid", "displacement"))
+        vel = iadd(vel, _pull_vel(vel, grid, interpolation=interpolation, bound=bound))
     if not displacement:
         vel = vel + ident
     return (vel, jac) if jacobian else vel
@@ -265,28 +266,27 @@ def exp_backward(vel, *gradhess, inverse=False, steps=8, interpolation="linear",
     grad = _matvec(ijac, grad)
     del ijac
     for _ in range(steps):
-        pulled = _pull_vel(grad, ident + vel, **opt)
+        grid = ident + vel    # the squaring's transformation, shared by its pulls
+        pulled = _pull_vel(grad, grid, **opt)
         step = _grad_step(jac, pulled, grad) if hess is None else None
         if step is not None:
             grad = step
-            jac = _composition_jac(jac, vel, type="disp", identity=ident, **opt)
-            vel += _pull_vel(vel, ident + vel, **opt)
-            continue
-        det = _det(jac)
-        jac = jac.transpose(-1, -2)
-        grad0 = grad
-        grad = _matvec(jac, pulled)
-        grad *= det[..., None]
-        grad += grad0
-        if hess is not None:
-            hess0 = hess
-            hess = _pull_vel(hess, ident + vel, **opt)
-            hess = _jhj(jac, hess)
-            hess *= det[..., None]
-            hess += hess0
-        jac = jac.transpose(-1, -2)
-        jac = _composition_jac(jac, vel, type="disp", identity=ident, **opt)
-        vel += _pull_vel(vel, ident + vel, **opt)
+        else:
+            det = _det(jac)
+            jac = jac.transpose(-1, -2)
+            grad0 = grad
+            grad = _matvec(jac, pulled)
+            grad *= det[..., None]
+            grad += grad0
+            if hess is not None:
+                hess0 = hess
+                hess = _pull_vel(hess, grid, **opt)
+                hess = _jhj(jac, hess)
+                hess *= det[..., None]
+                hess += hess0
+            jac = jac.transpose(-1, -2)
+        jac = _composition_jac(jac, grid, lhs=vel, type=("grid", "disp"), **opt)
+        vel += _pull_vel(vel, grid, **opt)
     if inverse:
         grad.neg_()
     grad /= (2 ** steps)
