@@ -389,7 +389,7 @@ def pcg_learned(net, hessian, gradient, penalty, voxel_size=(1, 1, 1), bound="df
         rz = _dot(r, z_new)
         z = z_new
         p.mul_(beta.clamp_min(0)).add_(z)
-    # fallback: FMG V-cycles from the current iterate (FMG judges the result)
+    # fallback: FMG V-cycles from the current iterate (their own residual test decides convergence)
     x2, rel, ncyc = SV.solve_to_tolerance(h, g, penalty, voxel_size, bound, tol=tol, nb_cycles=0, nb_iter=4,
                                           max_vcycles=fallback_cycles, max_levels=5, x=x, use_cg=True)
     return x2, rel, it, True
