@@ -265,6 +265,14 @@ int ni_precond_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *
 int ni_unet_conv3_fwd(int64_t N, const int64_t *shape, int32_t CI, int32_t CO, const void *x, const void *w,
                       const float *bias, void *y, int32_t act, int32_t out_f32, int32_t out_channels, void *stream);
 
+/* ni_unet_conv3_fwd on concat(nearest 2x upsampling of a [N, *shape/2, C1],
+ * skip [N, *shape, C2]) without materialising the concatenation (the result
+ * ni_unet_up2_concat + ni_unet_conv3_fwd give, bit for bit); bf16 output,
+ * act 0 / 1 as there, bias may be NULL, shape even.  Supported: 32 + 16 ->
+ * 16 (the decoder's first full-resolution layer). */
+int ni_unet_conv3_fwd_upcat(int64_t N, const int64_t *shape, int32_t C1, int32_t C2, int32_t CO, const void *a,
+                            const void *skip, const void *w, const float *bias, void *y, int32_t act, void *stream);
+
 /* dw[CO][KP] += sum over positions of dy (x) shifted x (fp32 accumulators,
  * caller zeroes them); db[CO] += sum of dy if db is not NULL. */
 int ni_unet_conv3_bwd_weight(int64_t N, const int64_t *shape, int32_t CI, int32_t CO, const void *x, const void *dy,

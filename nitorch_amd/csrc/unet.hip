@@ -288,10 +288,14 @@ struct SlotBytes {
   static constexpr int value = 2 * plane * 2 + KPad<CI>::value / 8 * 4;
 };
 
-template <int CI, int CO, int COS, int ACT, bool OUTF32>
+// C1 > 0: the input is concat(nearest 2x upsampling of X [N, D/2, H/2, W/2,
+// C1], X2 [N, D, H, W, CI - C1]) (up2_concat_fwd's output, D, H, W even),
+// staged from the two sources without materialising it.
+template <int CI, int CO, int COS, int ACT, bool OUTF32, int C1 = 0>
 __global__ __launch_bounds__(UN_BLOCK) void conv3_fwd_2slot(const bf16 *__restrict__ X, const bf16 *__restrict__ Wk,
                                                             const float *__restrict__ bias, void *__restrict__ Yv,
-                                                            int N, int D, int H, int W) {
+                                                            int N, int D, int H, int W,
+                                                            const bf16 *__restrict__ X2 = nullptr) {
   extern __shared__ __align__(16) unsigned char un_smem[];
   constexpr int PE = SlotBytes<CI>::plane;
   bf16 *box = (bf16 *)un_smem;
@@ -328,8 +332,17 @@ __global__ __launch_bounds__(UN_BLOCK) void conv3_fwd_2slot(const bf16 *__restri
       const int by = i / RC, j = i - by * RC;
       const int yy = y0 + by, zz = z0 + j / V8;
       v[u] = (uint4){0u, 0u, 0u, 0u};
-      if (xok && i < NCH && yy >= 0 && yy < H && zz >= 0 && zz < W)
-        v[u] = *(const uint4 *)(X + ((((int64_t)n * D + xx) * H + yy) * W + z0) * CI + j * 8);
+      if (xok && i < NCH && yy >= 0 && yy < H && zz >= 0 && zz < W) {
+        if constexpr (C1 == 0) {
+          v[u] = *(const uint4 *)(X + ((((int64_t)n * D + xx) * H + yy) * W + z0) * CI + j * 8);
+        } else {
+          const int c = (j % V8) * 8;
+          if (c < C1)
+            v[u] = *(const uint4 *)(X + ((((int64_t)n * (D / 2) + xx / 2) * (H / 2) + yy / 2) * (W / 2) + zz / 2) * C1 + c);
+          else
+            v[u] = *(const uint4 *)(X2 + ((((int64_t)n * D + xx) * H + yy) * W + zz) * (CI - C1) + (c - C1));
+        }
+      }
     }
   };
   auto store = [&](int slot) {
@@ -652,7 +665,7 @@ static inline unsigned nblk(int64_t n) { return (unsigned)((n + UN_BLOCK - 1) / 
 
 // Raise the dynamic-LDS limit of one staged-convolution instantiation to
 // `bytes` (once per device); false if that does not fit.
-template <int CI, int CO, int COS, int ACT, bool OUTF32, bool PLANE>    // conv3_fwd_plane, else conv3_fwd_2slot
+template <int CI, int CO, int COS, int ACT, bool OUTF32, bool PLANE, int C1 = 0>    // conv3_fwd_plane, else conv3_fwd_2slot
 static bool lds_ready() {
   constexpr int bytes = PLANE ? PlaneBytes<CI>::value : SlotBytes<CI>::value;
   static int state[64] = {0};    // 0 unknown, 1 usable, 2 not
@@ -663,7 +676,7 @@ static bool lds_ready() {
     bool ok = bytes <= 65536;
     const void *fn;
     if constexpr (PLANE) fn = (const void *)conv3_fwd_plane<CI, CO, COS, ACT, OUTF32>;
-    else fn = (const void *)conv3_fwd_2slot<CI, CO, COS, ACT, OUTF32>;
+    else fn = (const void *)conv3_fwd_2slot<CI, CO, COS, ACT, OUTF32, C1>;
     if (!ok && hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
         v >= bytes)
       ok = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess;
@@ -753,6 +766,33 @@ int ni_unet_conv3_fwd(int64_t N, const int64_t *shape, int32_t CI, int32_t CO, c
   UN_FWD_CASES(UN_X)
 #undef UN_X
   NI_CHECK(done, "unet_conv3_fwd: unsupported channel pair %d -> %d", CI, CO);
+  NI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int ni_unet_conv3_fwd_upcat(int64_t N, const int64_t *shape, int32_t C1, int32_t C2, int32_t CO, const void *a,
+                            const void *skip, const void *w, const float *bias, void *y, int32_t act, void *stream) {
+  set_error("");
+  NI_CHECK(a && skip && w && y && shape, "unet_conv3_fwd_upcat: missing argument");
+  NI_CHECK(C1 == 32 && C2 == 16 && CO == 16, "unet_conv3_fwd_upcat: only 32 + 16 -> 16 channels (got %d + %d -> %d)",
+           C1, C2, CO);
+  NI_CHECK(N >= 1 && shape[0] >= 2 && shape[1] >= 2 && shape[2] >= 2, "unet_conv3_fwd_upcat: bad shape");
+  NI_CHECK(shape[0] % 2 == 0 && shape[1] % 2 == 0 && shape[2] % 2 == 0, "unet_conv3_fwd_upcat: even sizes required");
+  const int D = (int)shape[0], H = (int)shape[1], W = (int)shape[2];
+  const int64_t tiles = N * D * ((H + UN_TY - 1) / UN_TY) * ((W + UN_TZ - 1) / UN_TZ);
+  NI_CHECK(tiles < (int64_t(1) << 31), "unet_conv3_fwd_upcat: too many tiles");
+  hipStream_t st = (hipStream_t)stream;
+  if (act) {
+    NI_CHECK((lds_ready<48, 16, 16, 1, false, false, 32>()), "unet_conv3_fwd_upcat: LDS limit not raised");
+    hipLaunchKernelGGL((conv3_fwd_2slot<48, 16, 16, 1, false, 32>), dim3((unsigned)tiles), dim3(UN_BLOCK),
+                       SlotBytes<48>::value, st, (const bf16 *)a, (const bf16 *)w, bias, y, (int)N, D, H, W,
+                       (const bf16 *)skip);
+  } else {
+    NI_CHECK((lds_ready<48, 16, 16, 0, false, false, 32>()), "unet_conv3_fwd_upcat: LDS limit not raised");
+    hipLaunchKernelGGL((conv3_fwd_2slot<48, 16, 16, 0, false, 32>), dim3((unsigned)tiles), dim3(UN_BLOCK),
+                       SlotBytes<48>::value, st, (const bf16 *)a, (const bf16 *)w, bias, y, (int)N, D, H, W,
+                       (const bf16 *)skip);
+  }
   NI_HIP_CHECK(hipGetLastError());
   return 0;
 }

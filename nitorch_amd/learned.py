@@ -36,6 +36,7 @@ _PF = ctypes.POINTER(ctypes.c_float)
 
 L.register_signatures({
     "ni_unet_conv3_fwd": [L._I64, L._PI64, L._I32, L._I32, L._P, L._P, L._P, L._P, L._I32, L._I32, L._I32, L._P],
+    "ni_unet_conv3_fwd_upcat": [L._I64, L._PI64, L._I32, L._I32, L._I32, L._P, L._P, L._P, L._P, L._P, L._I32, L._P],
     "ni_unet_conv3_bwd_weight": [L._I64, L._PI64, L._I32, L._I32, L._P, L._P, L._P, L._P, L._P],
     "ni_unet_lrelu_bwd": [L._I64, L._P, L._P, L._P, L._P],
     "ni_unet_avgpool2": [L._I32, L._I64, L._PI64, L._I32, L._P, L._P, L._P],
@@ -155,6 +156,16 @@ class UNet:
                                          int(act), int(out_f32), int(cos), L.stream_of(x)), "unet_conv3_fwd")
         return y
 
+    def _conv_upcat(self, a, skip, shape, c1, c2, co, w, bias, act):
+        """conv(concat(up2(a), skip)) in one pass (bf16 out)."""
+        N = a.shape[0]
+        D, H, W = shape
+        y = torch.empty(N, D, H, W, co, dtype=torch.bfloat16, device=a.device)
+        _check(L.lib().ni_unet_conv3_fwd_upcat(N, _shape(D, H, W), c1, c2, co, _ptr(a), _ptr(skip), _ptr(w),
+                                               _ptr(bias), _ptr(y), int(act), L.stream_of(a)),
+               "unet_conv3_fwd_upcat")
+        return y
+
     def _pool(self, x, shape, C):
         N, (D, H, W) = x.shape[0], shape
         y = torch.empty(N, D // 2, H // 2, W // 2, C, dtype=torch.bfloat16, device=x.device)
@@ -223,8 +234,12 @@ class UNet:
         c2 = self._upcat(a6, a4, s1, 64, 32)
         a7 = c(c2, s1, 96, 32, self.wk["d2a"], self.bias["d2a"], 1)
         a8 = c(a7, s1, 32, 32, self.wk["d2b"], self.bias["d2b"], 1)
-        c1 = self._upcat(a8, a2, s0, 32, 16)
-        a9 = c(c1, s0, 48, 16, self.wk["d1a"], self.bias["d1a"], 1)
+        if keep:    # backward needs the concatenation itself
+            c1 = self._upcat(a8, a2, s0, 32, 16)
+            a9 = c(c1, s0, 48, 16, self.wk["d1a"], self.bias["d1a"], 1)
+        else:
+            c1 = None
+            a9 = self._conv_upcat(a8, a2, s0, 32, 16, 16, self.wk["d1a"], self.bias["d1a"], 1)
         a10 = c(a9, s0, 16, 16, self.wk["d1b"], self.bias["d1b"], 1)
         out = c(a10, s0, 16, 16, self.wk["head"], self.bias["head"], 0, out_f32=True, cos=3)
         if keep:

@@ -196,3 +196,29 @@ def test_pcg_learned_plumbing(mod):
     assert rel_ <= 1e-4
     res = g - S.regulariser_grid(x.cpu().double().numpy(), h, **pen)
     assert np.linalg.norm(res) / np.linalg.norm(g) <= 2e-4
+
+
+def test_conv_upcat_matches_upcat_then_conv(mod):
+    """ni_unet_conv3_fwd_upcat == ni_unet_up2_concat + ni_unet_conv3_fwd, bit
+    for bit, and the network's inference forward (fused) equals its
+    training forward (concatenation materialised)."""
+    rng = np.random.default_rng(11)
+    net = mod.UNet(DEV)
+    for shape in [(4, 6, 18), (2, 10, 36), (6, 4, 66)]:
+        D, H, W = shape
+        a, _ = bf(rng.standard_normal((2, D // 2, H // 2, W // 2, 32)))
+        skip, _ = bf(rng.standard_normal((2,) + shape + (16,)))
+        w = torch.from_numpy(rng.standard_normal((16, 3, 3, 3, 48)) / np.sqrt(27 * 48)).float().to(DEV)
+        wk = mod.pack_weight(w, 48, 16)
+        b = torch.from_numpy(rng.standard_normal(16).astype(np.float32)).to(DEV)
+        for act in (0, 1):
+            ref = net._conv(net._upcat(a, skip, shape, 32, 16), shape, 48, 16, wk, b, act)
+            out = net._conv_upcat(a, skip, shape, 32, 16, 16, wk, b, act)
+            assert torch.equal(out, ref), (shape, act)
+    with torch.no_grad():
+        net.params["head.w"].normal_(0, 0.1)
+    net.refresh()
+    x, _ = bf(rng.standard_normal((1, 8, 8, 16, 16)))
+    out_inf = net.forward(x)
+    out_train, _ = net.forward(x, keep=True)
+    assert out_inf.abs().max() > 0 and torch.equal(out_inf, out_train)
