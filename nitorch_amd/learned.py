@@ -400,8 +400,9 @@ def pcg_learned(net, hessian, gradient, penalty, voxel_size=(1, 1, 1), bound="df
         if slow >= 2:
             break
         z_new = net.apply(r, h, penalty, voxel_size, bound, hscale)
-        beta = (_dot(r, z_new) - _dot(r, z)) / rz.clamp_min(1e-30)
-        rz = _dot(r, z_new)
+        rz_new = _dot(r, z_new)
+        beta = (rz_new - _dot(r, z)) / rz.clamp_min(1e-30)
+        rz = rz_new
         z = z_new
         p.mul_(beta.clamp_min(0)).add_(z)
     # fallback: FMG V-cycles from the current iterate (their own residual test decides convergence)
