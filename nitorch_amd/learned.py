@@ -373,40 +373,61 @@ def pcg_learned(net, hessian, gradient, penalty, voxel_size=(1, 1, 1), bound="df
     preconditioner (Polak-Ribiere beta, as the preconditioner is nonlinear).
     Channel-first contiguous fp32 tensors.  When the residual ratio of an
     iteration exceeds `stall` twice in a row, FMG V-cycles take over from
-    the current iterate.  Returns (x, relative residual, iterations, used_fallback)."""
+    the current iterate.  A non-finite value anywhere in the recurrence (a
+    network output with NaN / Inf, an overflowing step) counts as a stall:
+    the iterate is never updated with a non-finite step, and when the
+    network's warm start itself is not finite the fallback starts from zero.
+    Returns (x, relative residual, iterations, used_fallback)."""
     g, h = gradient, hessian
     gn = float(g.pow(2).sum().sqrt())
+    it = 0
     x = net.apply(g, h, penalty, voxel_size, bound, hscale) if warm_start else torch.zeros_like(g)
     r = SV.residual_grid_cf(x, g, h, penalty, voxel_size, bound)
     rn = float(r.pow(2).sum().sqrt())
-    if rn <= tol * gn:
+    fresh = not math.isfinite(rn) or not warm_start   # no usable iterate yet
+    if not math.isfinite(rn):
+        x = torch.zeros_like(g)
+    elif rn <= tol * gn:
         return x, rn / gn, 0, False
-    z = net.apply(r, h, penalty, voxel_size, bound, hscale)
-    p = z.clone()
-    rz = _dot(r, z)
-    Ap = torch.empty_like(g)
-    slow = 0
-    it = 0
-    for it in range(1, max_iter + 1):
-        SV.regulariser_grid_cf(p, h, penalty, voxel_size, bound, out=Ap)
-        alpha = rz / _dot(p, Ap).clamp_min(1e-30)
-        x.add_(p * alpha)
-        r.sub_(Ap * alpha)
-        rn_new = float(r.pow(2).sum().sqrt())
-        if rn_new <= tol * gn:
-            return x, rn_new / gn, it, False
-        slow = slow + 1 if rn_new > stall * rn else 0
-        rn = rn_new
-        if slow >= 2:
-            break
-        z_new = net.apply(r, h, penalty, voxel_size, bound, hscale)
-        rz_new = _dot(r, z_new)
-        beta = (rz_new - _dot(r, z)) / rz.clamp_min(1e-30)
-        rz = rz_new
-        z = z_new
-        p.mul_(beta.clamp_min(0)).add_(z)
-    # fallback: FMG V-cycles from the current iterate (their own residual test decides convergence)
-    x2, rel, ncyc = SV.solve_to_tolerance(h, g, penalty, voxel_size, bound, tol=tol, nb_cycles=0, nb_iter=4,
+    else:
+        z = net.apply(r, h, penalty, voxel_size, bound, hscale)
+        p = z.clone()
+        rz = _dot(r, z)
+        Ap = torch.empty_like(g)
+        slow = 0
+        for it in range(1, max_iter + 1):
+            SV.regulariser_grid_cf(p, h, penalty, voxel_size, bound, out=Ap)
+            pap, rzf = float(_dot(p, Ap)), float(rz)
+            if not (math.isfinite(pap) and math.isfinite(rzf)):
+                break                      # the direction is unusable; x is untouched
+            alpha = rzf / max(pap, 1e-30)
+            x.add_(p, alpha=alpha)
+            r.sub_(Ap, alpha=alpha)
+            fresh = False
+            rn_new = float(r.pow(2).sum().sqrt())
+            if not math.isfinite(rn_new):
+                x.sub_(p, alpha=alpha)     # back to the last finite iterate
+                fresh = not warm_start and it == 1
+                break
+            if rn_new <= tol * gn:
+                return x, rn_new / gn, it, False
+            slow = slow + 1 if rn_new > stall * rn else 0
+            rn = rn_new
+            if slow >= 2:
+                break
+            z_new = net.apply(r, h, penalty, voxel_size, bound, hscale)
+            rz_new = _dot(r, z_new)
+            beta = (rz_new - _dot(r, z)) / rz.clamp_min(1e-30)
+            rz = rz_new
+            z = z_new
+            p.mul_(beta.clamp_min(0)).add_(z)
+        if not bool(torch.isfinite(x).all()):
+            fresh = True
+            x = torch.zeros_like(g)
+    # fallback: FMG V-cycles from the current iterate (their own residual test
+    # decides convergence); a full FMG pass first when starting from zero
+    x2, rel, ncyc = SV.solve_to_tolerance(h, g, penalty, voxel_size, bound, tol=tol, nb_cycles=2 if fresh else 0,
+                                          nb_iter=4,
                                           max_vcycles=fallback_cycles, max_levels=5, x=x, use_cg=True)
     return x2, rel, it, True
 

@@ -270,3 +270,104 @@ def test_oracle_vs_reference_random():
         r = m.grid_push(torch.from_numpy(y), torch.from_numpy(g), list(sp), R.bounds([bound]),
                         R.orders([order]), 0).numpy()
         assert rel(O.grid_push(y, g, sp, bound, order, 0), r) < 1e-12
+
+
+GOLDEN_EXTRA = os.path.join(HERE, "golden", "pushpull_golden_extra.npz")
+
+
+def test_golden_extrapolate2_and_grad_backward_input():
+    """Oracle == reference on the extrapolate=2 band (tolerance 0.55) for every
+    entry point, and on grid_grad_backward's input branch at extrapolate
+    0 / 1 / 2 (tests/golden/make_golden_extra.py)."""
+    g = np.load(GOLDEN_EXTRA)
+    n = n_band = 0
+    for key, c in _cases(g):
+        dim, order, b, extrap, f64 = [int(v) for v in c["meta"]]
+        assert extrap == 2
+        bound = BOUNDS[b]
+        tol = 1e-12 if f64 else 2e-6
+        inp, grid = c["inp"], c["grid"]
+        shape = inp.shape[2:]
+        # the grid fills the band between the two tolerances on some axis
+        sz = np.array(shape, np.float64)
+        band = np.any(((grid >= -0.55) & (grid < -0.05)) | ((grid >= sz - 1 + 0.05) & (grid < sz - 1 + 0.55)), -1)
+        n_band += int(band.sum())
+        checks = {
+            "pull": lambda: O.grid_pull(inp, grid, bound, order, 2),
+            "push": lambda: O.grid_push(c["srcpush"], grid, shape, bound, order, 2),
+            "count": lambda: O.grid_count(grid, shape, bound, order, 2),
+            "pbw_in": lambda: O.grid_pull_backward(c["gpull"], inp, grid, bound, order, 2, need_grid=False)[0],
+            "ubw_in": lambda: O.grid_push_backward(inp, c["srcpush"], grid, bound, order, 2, need_grid=False)[0],
+        }
+        if "sgrad" in c:
+            checks["sgrad"] = lambda: O.grid_grad(inp, grid, bound, order, 2)
+        if "pbw_grid" in c:
+            checks["pbw_grid"] = lambda: O.grid_pull_backward(c["gpull"], inp, grid, bound, order, 2)[1]
+            checks["ubw_grid"] = lambda: O.grid_push_backward(inp, c["srcpush"], grid, bound, order, 2)[1]
+            checks["cbw_grid"] = lambda: O.grid_count_backward(c["gcount"], grid, bound, order, 2)
+        for ex in (0, 1, 2):
+            if f"gbw_in{ex}" in c:
+                checks[f"gbw_in{ex}"] = (lambda ex=ex: O.grid_grad_backward(c["ggrad"], inp, grid, bound, order, ex,
+                                                                          need_grid=False)[0])
+        for name, fn in checks.items():
+            ref = c[name]
+            e = rel(fn(), ref) if np.abs(ref).max() > 0 else float(np.abs(fn()).max())
+            assert e <= tol, (key, name, dim, order, bound, f64, e)
+            n += 1
+    assert n > 2000 and n_band > 500, (n, n_band)
+
+
+def test_extrapolate2_tolerance_closed_form():
+    """The widened tolerance on its own: a sample 0.3 voxel outside the first
+    or last voxel is out of bounds at extrapolate=0 (pull gives 0) and in
+    bounds at extrapolate=2, where linear interpolation with replicate
+    padding returns the edge value exactly."""
+    x = np.arange(1.0, 7.0).reshape(1, 1, 6)
+    grid = np.array([-0.3, 5.3, -0.6, 2.5]).reshape(1, 4, 1)
+    p0 = O.grid_pull(x, grid, "replicate", 1, 0)[0, 0]
+    p2 = O.grid_pull(x, grid, "replicate", 1, 2)[0, 0]
+    assert p0[0] == 0 and p0[1] == 0 and p0[3] == 3.5
+    assert p2[0] == 1.0 and p2[1] == 6.0 and p2[2] == 0 and p2[3] == 3.5
+
+
+@pytest.mark.parametrize("order", range(8))
+@pytest.mark.parametrize("bound", BOUNDS)
+def test_grad_grad_backward_input_adjoint(order, bound):
+    """<grid_grad(x, g), y> = <x, grid_grad_backward(y, x, g)[input]> in FP64:
+    the input branch is the exact adjoint of grid_grad (a dropped tap, a wrong
+    sign or a transposed component breaks it), for extrapolate 0 / 1 / 2."""
+    rng = np.random.default_rng(100 + order * 7 + BOUNDS.index(bound))
+    x = rng.standard_normal((2, 3, 5, 6, 4))
+    g = _identity((4, 3, 5)) + 1.5 * rng.standard_normal((2, 4, 3, 5, 3))
+    y = rng.standard_normal((2, 3, 4, 3, 5, 3))
+    for extrap in (0, 1, 2):
+        lhs = np.sum(O.grid_grad(x, g, bound, order, extrap) * y)
+        gi, _ = O.grid_grad_backward(y, x, g, bound, order, extrap, need_grid=False)
+        rhs = np.sum(x * gi)
+        assert abs(lhs - rhs) <= 1e-12 * max(1, np.abs(y).sum()), (extrap, lhs, rhs)
+        if order == 0:
+            assert np.abs(gi).max() == 0
+
+
+def test_grad_grad_backward_input_finite_differences():
+    """grid_grad is linear in its input, so the input branch equals the
+    directional derivative of <grid_grad(x, g), y> along any input direction,
+    and grid_grad itself equals central differences of pull along each grid
+    axis (orders >= 2, where the spline derivative is continuous)."""
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((1, 2, 7, 6, 5))
+    g = _identity((3, 3, 3)) + 1.5 + 0.3 * rng.standard_normal((1, 3, 3, 3, 3))
+    y = rng.standard_normal((1, 2, 3, 3, 3, 3))
+    for order in (2, 3, 5, 7):
+        gi, _ = O.grid_grad_backward(y, x, g, "dct2", order, 1, need_grid=False)
+        d = rng.standard_normal(x.shape)
+        h = 1e-3
+        fp = np.sum(O.grid_grad(x + h * d, g, "dct2", order, 1) * y)
+        fm = np.sum(O.grid_grad(x - h * d, g, "dct2", order, 1) * y)
+        assert abs((fp - fm) / (2 * h) - np.sum(gi * d)) < 1e-9 * max(1, abs(np.sum(gi * d)))
+        sg = O.grid_grad(x, g, "dct2", order, 1)
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = 1e-5
+            fd = (O.grid_pull(x, g + e, "dct2", order, 1) - O.grid_pull(x, g - e, "dct2", order, 1)) / 2e-5
+            assert np.abs(fd - sg[..., k]).max() < 1e-6 * max(1, np.abs(sg).max()), (order, k)
