@@ -67,38 +67,74 @@ def affine_lmdiv(a, b):
     return affine_matmul(affine_inv(a), b)
 
 
-def affine_resize(affine, shape, factor, anchor="c", shape_out=None):
-    """Affine of a resized lattice (nitorch/spatial/_affine.py:1712-1790):
-    returns (affine, shape_out)."""
-    affine = torch.as_tensor(affine)
-    nb_dim = affine.shape[-1] - 1
-    factor = list(factor) if isinstance(factor, (list, tuple)) else [factor]
-    factor = [float(f) for f in (factor + factor[-1:] * nb_dim)[:nb_dim]]
-    anchor = list(anchor) if isinstance(anchor, (list, tuple)) else [anchor]
-    anchor = [a[0].lower() for a in (anchor + anchor[-1:] * nb_dim)[:nb_dim]]
-    if shape_out is None:
-        shape_out = [max(1, int(round(s * f))) for s, f in zip(shape, factor)]
-    shifts, scales = [], []
-    for anch, f, inshp, outshp in zip(anchor, factor, shape, shape_out):
-        if inshp == 1 or outshp == 1:
-            anch = "e"
-        if anch == "c":
-            shifts.append(0.)
-            scales.append((inshp - 1) / (outshp - 1))
-        elif anch == "e":
-            shifts.append(0.5 * (inshp / outshp - 1))
-            scales.append(inshp / outshp)
-        elif anch == "f":
-            shifts.append(0.)
-            scales.append(1 / f)
-        elif anch == "l":
-            shifts.append((inshp - 1) - (outshp - 1) / f)
-            scales.append(1 / f)
+class LatticeAxis:
+    """One axis of a resized lattice: output voxel o sits at input voxel
+    coordinate `scale * o + shift`.
+
+    The anchor decides which points of the two lattices coincide
+    (nitorch/spatial/_affine.py:1712-1790, nitorch/spatial/_grid.py:618-754):
+    'c' the centres of the first and last voxels, 'e' the outer edges of the
+    volume, 'f' the centre of the first voxel and 'l' that of the last one,
+    the last two with the voxel size divided exactly by `factor`.  An axis
+    of length 1 on either side is always edge-anchored, and when both sides
+    have length 1 the scale is 1 / factor.
+    """
+
+    __slots__ = ("anchor", "factor", "n_in", "n_out", "scale", "shift")
+
+    def __init__(self, anchor, factor, n_in, n_out):
+        anchor = anchor[0].lower()
+        if anchor not in "cefl":
+            raise ValueError(f"Unknown anchor {anchor}")
+        if n_in == 1 or n_out == 1:
+            anchor = "e"
+        self.anchor, self.factor, self.n_in, self.n_out = anchor, factor, n_in, n_out
+        if anchor == "c":
+            self.scale, self.shift = (n_in - 1) / (n_out - 1), 0.
+        elif anchor == "e":
+            self.scale = n_in / n_out
+            self.shift = 0.5 * (self.scale - 1)
         else:
-            raise ValueError("Unknown anchor {}".format(anch))
-        if inshp == outshp == 1:
-            scales[-1] = 1 / f
-    lin = torch.diag(torch.as_tensor(scales, dtype=affine.dtype, device=affine.device))
-    trl = torch.as_tensor(shifts, dtype=affine.dtype, device=affine.device)[..., None]
-    trf = torch.cat((lin, trl), dim=1)
-    return affine_matmul(affine, trf), tuple(shape_out)
+            self.scale = 1 / factor
+            self.shift = 0. if anchor == "f" else (n_in - 1) - (n_out - 1) / factor
+        if n_in == n_out == 1:
+            self.scale = 1 / factor
+
+    def coordinates(self, dtype=None, device=None):
+        """Input coordinates of the output voxels, as a 1-D tensor.  The
+        arithmetic follows the reference's so that sampled values agree to
+        the last bit: `linspace` for centres, a product with the scale for
+        edges, a division by the factor for first / last."""
+        if self.anchor == "c":
+            return torch.linspace(0, self.n_in - 1, self.n_out, dtype=dtype, device=device)
+        o = torch.arange(0., self.n_out, dtype=dtype, device=device)
+        if self.anchor == "e":
+            return o * self.scale + self.shift
+        o = o / self.factor
+        return o + self.shift if self.anchor == "l" else o
+
+
+def _per_axis(values, n):
+    """A value or a sequence, extended to `n` entries by repeating its last."""
+    values = list(values) if isinstance(values, (list, tuple)) else [values]
+    return values + values[-1:] * (n - len(values)) if len(values) < n else values[:n]
+
+
+def affine_resize(affine, shape, factor, anchor="c", shape_out=None):
+    """Orientation matrix of a resized lattice, and the resized shape
+    (nitorch/spatial/_affine.py:1712-1790).  The new voxel-to-world matrix
+    is `affine @ [diag(scale) | shift]`, with each axis's scale and shift
+    from `LatticeAxis`; `shape_out` defaults to round(shape * factor), at
+    least 1."""
+    affine = torch.as_tensor(affine)
+    ndim = affine.shape[-1] - 1
+    factor = [float(f) for f in _per_axis(factor, ndim)]
+    anchor = _per_axis(anchor, ndim)
+    if shape_out is None:
+        shape_out = [max(1, int(round(n * f))) for n, f in zip(shape, factor)]
+    axes = [LatticeAxis(a, f, n, m) for a, f, n, m in zip(anchor, factor, shape, shape_out)]
+    voxel_map = torch.zeros((ndim, ndim + 1), dtype=affine.dtype, device=affine.device)
+    for d, ax in enumerate(axes):
+        voxel_map[d, d] = ax.scale
+        voxel_map[d, ndim] = ax.shift
+    return affine_matmul(affine, voxel_map), tuple(shape_out)

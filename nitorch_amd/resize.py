@@ -36,40 +36,36 @@ class GridAlignType(enum.IntEnum):
 
 
 def make_list(x, n=None):
-    if not isinstance(x, (list, tuple)):
-        x = [x]
-    x = list(x)
-    if n:
-        x = x + max(0, n - len(x)) * x[-1:]
-    return x
+    """A value or a sequence as a list, extended to `n` entries by repeating
+    its last one."""
+    out = list(x) if isinstance(x, (list, tuple)) else [x]
+    if n and len(out) < n:
+        out += out[-1:] * (n - len(out))
+    return out
+
+
+_ALIGN_NAMES = {"c": "center", "e": "edge", "f": "first", "l": "last"}
 
 
 def align_to_nitorch(align, as_type="str"):
-    """Normalise alignment modes (nitorch/_C/resize.py:16-53)."""
-    intype = type(align)
-    align = make_list(align)
-    oalign = []
-    for b in align:
-        if isinstance(b, GridAlignType):
-            b = b.name
-        b = b.lower() if isinstance(b, str) else b
-        if b[0] == "c":
-            oalign.append("center")
-        elif b[0] == "e":
-            oalign.append("edge")
-        elif b[0] == "f":
-            oalign.append("first")
-        elif b[0] == "l":
-            oalign.append("last")
-        else:
-            raise ValueError(f"Unknown boundary condition {b}")
-    if as_type in ("enum", "int", int):
-        oalign = [getattr(GridAlignType, b) for b in oalign]
-    if issubclass(intype, (list, tuple)):
-        oalign = intype(oalign)
-    else:
-        oalign = oalign[0]
-    return oalign
+    """Normalise grid-alignment modes (nitorch/_C/resize.py:16-53): each mode
+    is a `GridAlignType` or a string whose first letter is c / e / f / l
+    ('centers', 'edges', 'first', 'last').  Returns names, or enum members
+    when `as_type` is 'enum' / 'int' / int; a list or tuple in gives the same
+    container out, a single mode a single value."""
+    def name_of(mode):
+        if isinstance(mode, GridAlignType):
+            return mode.name
+        key = mode.lower()[:1] if isinstance(mode, str) else None
+        if key not in _ALIGN_NAMES:
+            raise ValueError(f"Unknown alignment mode {mode}")
+        return _ALIGN_NAMES[key]
+
+    as_enum = as_type in ("enum", "int", int)
+    convert = (lambda m: GridAlignType[name_of(m)]) if as_enum else name_of
+    if isinstance(align, (list, tuple)):
+        return type(align)(convert(m) for m in align)
+    return convert(align)
 
 
 def _pad3(vals):
@@ -165,20 +161,20 @@ def c_resize(input, factor=None, bound="dct2", interpolation=1, mode="center",
              shape=None, output=None, adjoint=False, normalize=False):
     """Resize a spatial tensor (nitorch/_C/resize.py:64-97).
 
-    input : (N, C, *inshape) tensor; factor, bound, interpolation, mode:
-    [sequence of] values, padded by repeating the last one.
-    Returns (N, C, *shape)."""
-    bound = bound_to_nitorch(make_list(bound), "int")
-    interpolation = inter_to_nitorch(make_list(interpolation), "int")
-    mode = align_to_nitorch(make_list(mode), "int")
-    if output is None and shape:
-        output = input.new_zeros([*input.shape[:2], *shape])
+    input : (N, C, *inshape) tensor.  `factor`, `bound`, `interpolation`
+    and `mode` are values or per-dimension sequences (the last entry
+    repeats).  With `shape` (and no `output`), the result is allocated at
+    that shape; otherwise its shape follows from the factor.  At least one
+    of `factor` and `shape` is required.  Returns (N, C, *shape)."""
     if torch.is_tensor(factor):
         factor = factor.double().tolist()
     if not factor and not shape:
-        raise ValueError("At least one of factor or shape must be provided")
-    factor = make_list(factor or [1.])
-    return _resize(input, output, factor, bound, interpolation, mode, adjoint, normalize)
+        raise ValueError("c_resize: give a factor, a shape or both")
+    if output is None and shape:
+        output = input.new_zeros(tuple(input.shape[:2]) + tuple(shape))
+    return _resize(input, output, make_list(factor or 1.), bound_to_nitorch(make_list(bound), "int"),
+                   inter_to_nitorch(make_list(interpolation), "int"), align_to_nitorch(make_list(mode), "int"),
+                   adjoint, normalize)
 
 
 def c_prolongation(input, factor=2., bound="dct2", shape=None, output=None):

@@ -20,54 +20,83 @@ __all__ = ["grid_pull", "grid_push", "grid_count", "grid_grad",
            "BoundType", "InterpolationType"]
 
 
-def _expanded_shape(*shapes):
-    """Broadcast shapes right-aligned (like torch.broadcast_shapes)."""
-    return tuple(torch.broadcast_shapes(*[tuple(s) for s in shapes]))
+class _Layout:
+    """Where the caller's leading (batch) and channel dimensions go.
+
+    The backend works on `(B, C, *spatial)` images and `(B, *spatial, D)`
+    grids.  `lead` holds the broadcast leading dimensions that were folded
+    into B, and `chan` the channel dimensions put back on the result: the
+    input's own channel size when it had one, else a singleton channel when
+    there were leading dimensions, else none."""
+
+    __slots__ = ("lead", "chan", "dim")
+
+    def __init__(self, lead, chan, dim):
+        self.lead, self.chan, self.dim = tuple(lead), tuple(chan), dim
+
+    def restore(self, out, vector=False):
+        """(B, C, *spatial[, D]) backend result -> caller layout."""
+        tail = tuple(out.shape[out.dim() - 1:]) if vector else ()
+        stop = out.dim() - len(tail)
+        return out.reshape(self.lead + self.chan + tuple(out.shape[stop - self.dim:stop]) + tail)
 
 
-def _preproc(grid, input=None, mode=None):
-    """Broadcast / reshape to the backend's [B, C, *spatial] / [B, *spatial, D]
-    (nitorch/spatial/_grid.py:90-127)."""
+def _fold_grid(grid):
+    """(..., *spatial, D) grid alone -> (B, *spatial, D) and its layout
+    (nitorch/spatial/_grid.py:90-127, the count branch)."""
     dim = grid.shape[-1]
-    if input is None:
-        spatial = grid.shape[-dim - 1:-1]
-        batch = grid.shape[:-dim - 1]
-        grid = grid.reshape([-1, *spatial, dim])
-        info = dict(batch=batch, channel=[1] if batch else [], dim=dim)
-        return grid, info
-
-    grid_spatial = grid.shape[-dim - 1:-1]
-    grid_batch = grid.shape[:-dim - 1]
-    input_spatial = input.shape[-dim:]
-    channel = 0 if input.dim() == dim else input.shape[-dim - 1]
-    input_batch = input.shape[:-dim - 1]
-
-    if mode == "push":
-        grid_spatial = input_spatial = _expanded_shape(grid_spatial, input_spatial)
-
-    batch = _expanded_shape(grid_batch, input_batch)
-    grid = grid.expand([*batch, *grid_spatial, dim])
-    grid = grid.reshape([-1, *grid_spatial, dim])
-    input = input.expand([*batch, channel or 1, *input_spatial])
-    input = input.reshape([-1, channel or 1, *input_spatial])
-
-    out_channel = [channel] if channel else ([1] if batch else [])
-    info = dict(batch=batch, channel=out_channel, dim=dim)
-    return grid, input, info
+    lead = tuple(grid.shape[:grid.dim() - dim - 1])
+    folded = grid.reshape((-1,) + tuple(grid.shape[grid.dim() - dim - 1:]))
+    return folded, _Layout(lead, (1,) if lead else (), dim)
 
 
-def _postproc(out, shape_info, mode):
-    """nitorch/spatial/_grid.py:130-143"""
-    dim = shape_info["dim"]
-    if mode != "grad":
-        spatial = out.shape[-dim:]
-        feat = []
-    else:
-        spatial = out.shape[-dim - 1:-1]
-        feat = [out.shape[-1]]
-    batch = shape_info["batch"]
-    channel = shape_info["channel"]
-    return out.reshape([*batch, *channel, *spatial, *feat])
+def _fold_pair(grid, image, shared_space=False):
+    """Broadcast the leading dimensions of a grid (..., *gspatial, D) and an
+    image (..., [C], *ispatial) and fold them into one batch dimension
+    (nitorch/spatial/_grid.py:90-127).  With `shared_space` (push) the two
+    spatial shapes are broadcast against each other as well.  Returns the
+    (B, *spatial, D) grid, the (B, C, *spatial) image and the layout."""
+    dim = grid.shape[-1]
+    gspace = tuple(grid.shape[grid.dim() - dim - 1:-1])
+    ispace = tuple(image.shape[image.dim() - dim:])
+    has_chan = image.dim() > dim
+    nchan = image.shape[image.dim() - dim - 1] if has_chan else 0
+    ilead = tuple(image.shape[:image.dim() - dim - 1]) if has_chan else ()
+    glead = tuple(grid.shape[:grid.dim() - dim - 1])
+    if shared_space:
+        gspace = ispace = tuple(torch.broadcast_shapes(gspace, ispace))
+    lead = tuple(torch.broadcast_shapes(glead, ilead))
+    width = nchan if nchan else 1
+    grid = grid.expand(lead + gspace + (dim,)).reshape((-1,) + gspace + (dim,))
+    image = image.expand(lead + (width,) + ispace).reshape((-1, width) + ispace)
+    chan = (nchan,) if nchan else ((1,) if lead else ())
+    return grid, image, _Layout(lead, chan, dim)
+
+
+def _pull_labels(labels, grid, interpolation, bound, extrapolate, prefilter):
+    """Label-map pull (nitorch/spatial/_grid.py:187-198): every label value
+    is pulled as a soft indicator, and each output voxel keeps the label
+    whose pulled indicator is strictly largest (the first label wins ties,
+    voxels where every indicator is <= 0 keep 0).
+
+    With `prefilter`, the reference assigns the in-place prefiltered
+    indicator of one label to the image that the next labels' indicators
+    are compared with (_grid.py:193-195); that statement is reproduced so
+    that label maps resample as they do there."""
+    dim = grid.shape[-1]
+    shape = tuple(labels.shape[:2]) + tuple(grid.shape[1:-1])
+    winner = labels.new_zeros(shape)
+    best = grid.new_zeros(shape)
+    source = labels
+    for value in labels.unique():
+        mask = (source == value).to(grid.dtype)
+        if prefilter:
+            source = spline_coeff_nd(mask, interpolation=interpolation, bound=bound, dim=dim,
+                                     inplace=True)
+        score = GridPull.apply(mask, grid, interpolation, bound, extrapolate)
+        winner.masked_fill_(score > best, value)
+        torch.maximum(best, score, out=best)
+    return winner
 
 
 def grid_pull(input, grid, interpolation="linear", bound="zero",
@@ -79,58 +108,40 @@ def grid_pull(input, grid, interpolation="linear", bound="zero",
     grid  : (..., *outshape, dim) tensor of voxel coordinates
     returns (..., [channel], *outshape)
 
-    Integer inputs are treated as label maps: each label is pulled as a soft
-    mask and the arg-max label is returned.  With `prefilter`, the input is
-    first turned into interpolating spline coefficients (`spline_coeff_nd`).
-    For label maps this keeps the reference's statement
-    `input = spline_coeff_nd(soft, ..., inplace=True)` (_grid.py:193-195):
-    the prefiltered mask of one label replaces `input` for the next labels.
-    """
-    grid, input, shape_info = _preproc(grid, input)
-    batch, channel = input.shape[:2]
-    dim = grid.shape[-1]
-
-    if not input.dtype.is_floating_point:
-        out = input.new_zeros([batch, channel, *grid.shape[1:-1]])
-        pmax = grid.new_zeros([batch, channel, *grid.shape[1:-1]])
-        for label in input.unique():
-            soft = (input == label).to(grid.dtype)
-            if prefilter:
-                input = spline_coeff_nd(soft, interpolation=interpolation, bound=bound, dim=dim,
-                                        inplace=True)
-            soft = GridPull.apply(soft, grid, interpolation, bound, extrapolate)
-            out[soft > pmax] = label
-            pmax = torch.max(pmax, soft)
-    else:
-        if prefilter:
-            input = spline_coeff_nd(input, interpolation=interpolation, bound=bound, dim=dim)
-        out = GridPull.apply(input, grid, interpolation, bound, extrapolate)
-    return _postproc(out, shape_info, mode="pull")
+    Integer inputs are label maps (see `_pull_labels`).  With `prefilter`,
+    the input is first turned into interpolating spline coefficients
+    (`spline_coeff_nd`)."""
+    grid, image, layout = _fold_pair(grid, input)
+    if not image.dtype.is_floating_point:
+        return layout.restore(_pull_labels(image, grid, interpolation, bound, extrapolate, prefilter))
+    if prefilter:
+        image = spline_coeff_nd(image, interpolation=interpolation, bound=bound, dim=layout.dim)
+    return layout.restore(GridPull.apply(image, grid, interpolation, bound, extrapolate))
 
 
 def grid_push(input, grid, shape=None, interpolation="linear", bound="zero",
               extrapolate=False, prefilter=False):
     """Splat an image with respect to a deformation field (adjoint of pull,
-    nitorch/spatial/_grid.py:208-251); with `prefilter`, the splatted image is
-    turned into interpolating spline coefficients."""
-    grid, input, shape_info = _preproc(grid, input, mode="push")
-    dim = grid.shape[-1]
-    if shape is None:
-        shape = tuple(input.shape[2:])
-    out = GridPush.apply(input, grid, shape, interpolation, bound, extrapolate)
+    nitorch/spatial/_grid.py:208-251).  The output has `shape` (default: the
+    broadcast spatial shape of input and grid); with `prefilter`, it is
+    turned into interpolating spline coefficients in place."""
+    grid, image, layout = _fold_pair(grid, input, shared_space=True)
+    shape = tuple(image.shape[2:]) if shape is None else shape
+    out = GridPush.apply(image, grid, shape, interpolation, bound, extrapolate)
     if prefilter:
-        out = spline_coeff_nd(out, interpolation=interpolation, bound=bound, dim=dim, inplace=True)
-    return _postproc(out, shape_info, mode="push")
+        out = spline_coeff_nd(out, interpolation=interpolation, bound=bound, dim=layout.dim,
+                              inplace=True)
+    return layout.restore(out)
 
 
 def grid_count(grid, shape=None, interpolation="linear", bound="zero",
                extrapolate=False):
-    """Splatting weights (push of ones, nitorch/spatial/_grid.py:254-287)."""
-    grid, shape_info = _preproc(grid)
-    if shape is None:
-        shape = tuple(grid.shape[1:-1])
-    out = GridCount.apply(grid, shape, interpolation, bound, extrapolate)
-    return _postproc(out, shape_info, mode="count")
+    """Splatting weights, i.e. the push of an image of ones
+    (nitorch/spatial/_grid.py:254-287); output shape `shape`, default the
+    grid's spatial shape."""
+    grid, layout = _fold_grid(grid)
+    shape = tuple(grid.shape[1:-1]) if shape is None else shape
+    return layout.restore(GridCount.apply(grid, shape, interpolation, bound, extrapolate))
 
 
 def grid_grad(input, grid, interpolation="linear", bound="zero",
@@ -139,12 +150,10 @@ def grid_grad(input, grid, interpolation="linear", bound="zero",
     (nitorch/spatial/_grid.py:290-328); returns (..., [channel], *outshape, dim).
     With `prefilter`, the input is first turned into interpolating spline
     coefficients."""
-    grid, input, shape_info = _preproc(grid, input)
-    dim = grid.shape[-1]
+    grid, image, layout = _fold_pair(grid, input)
     if prefilter:
-        input = spline_coeff_nd(input, interpolation=interpolation, bound=bound, dim=dim)
-    out = GridGrad.apply(input, grid, interpolation, bound, extrapolate)
-    return _postproc(out, shape_info, mode="grad")
+        image = spline_coeff_nd(image, interpolation=interpolation, bound=bound, dim=layout.dim)
+    return layout.restore(GridGrad.apply(image, grid, interpolation, bound, extrapolate), vector=True)
 
 
 def spline_coeff(input, interpolation="linear", bound="dct2", dim=-1, inplace=False):
@@ -212,169 +221,122 @@ def _matvec(mat, vec):
     return (mat * vec[..., None, :]).sum(-1)
 
 
-def _pad_last(vals, n):
-    vals = list(vals)
-    return (vals + vals[-1:] * n)[:n]
-
-
 def affine_grid(mat, shape, jitter=False):
-    """Grid of the voxel coordinates mat @ [i, j, k, 1] of a lattice of
-    `shape` (nitorch/spatial/_grid.py:579-615): (*batch, *shape, dim)."""
+    """Voxel coordinates `A @ [i, j, k] + t` of the lattice `shape` for an
+    affine matrix [A | t] of shape (..., D[+1], D+1)
+    (nitorch/spatial/_grid.py:579-615); returns (..., *shape, D), the
+    matrix's leading dimensions first."""
     mat = torch.as_tensor(mat)
-    shape = list(shape)
-    nb_dim = mat.shape[-1] - 1
-    if nb_dim != len(shape):
-        raise ValueError("Dimension of the affine matrix ({}) and shape ({}) "
-                         "are not the same.".format(nb_dim, len(shape)))
-    if mat.shape[-2] not in (nb_dim, nb_dim + 1):
-        raise ValueError("First argument should be matrces of shape (..., {0}, {1}) or "
-                         "(..., {1}, {1}) but got {2}.".format(nb_dim, nb_dim + 1, mat.shape))
-    batch_shape = mat.shape[:-2]
-    grid = identity_grid(shape, mat.dtype, mat.device, jitter=jitter)
-    if batch_shape:
-        grid = grid.reshape([1] * len(batch_shape) + list(grid.shape))
-        mat = mat.reshape(list(batch_shape) + [1] * nb_dim + list(mat.shape[-2:]))
-    return _matvec(mat[..., :nb_dim, :nb_dim], grid) + mat[..., :nb_dim, -1]
+    ndim = mat.shape[-1] - 1
+    if len(shape) != ndim:
+        raise ValueError(f"affine_grid: a ({ndim}+1)-column matrix needs a {ndim}-D shape, got {list(shape)}")
+    if mat.shape[-2] not in (ndim, ndim + 1):
+        raise ValueError(f"affine_grid: expected (..., {ndim}[+1], {ndim + 1}) matrices, got {tuple(mat.shape)}")
+    lead = tuple(mat.shape[:-2])
+    ijk = identity_grid(list(shape), mat.dtype, mat.device, jitter=jitter)
+    ijk = ijk.reshape((1,) * len(lead) + tuple(ijk.shape))
+    lin = mat[..., :ndim, :ndim].reshape(lead + (1,) * ndim + (ndim, ndim))
+    off = mat[..., :ndim, ndim].reshape(lead + (1,) * ndim + (ndim,))
+    return _matvec(lin, ijk) + off
+
+
+def _resize_lattice(image, factor, shape, anchor, dim):
+    """The output shape and the per-axis `LatticeAxis` maps of a resize.
+
+    The output length of an axis is `shape` when given, else
+    max(1, int(n * factor)); a factor left unset is output / input length
+    (nitorch/spatial/_grid.py:693-700)."""
+    from .affine import LatticeAxis, _per_axis
+    if dim is None:
+        dim = len(shape) if shape is not None else image.dim() - 2
+    n_in = list(image.shape[image.dim() - dim:])
+    if torch.is_tensor(factor):
+        factor = factor.tolist()
+    factor = [float(f or 0.) for f in _per_axis(factor if factor is not None else 0., dim)]
+    wanted = _per_axis(shape if shape is not None else [None], dim)
+    n_out = [m or max(1, int(n * f)) for n, m, f in zip(n_in, wanted, factor)]
+    if not all(n_out):
+        raise ValueError("Either factor or shape must be set in all dimensions")
+    factor = [f or m / n for n, m, f in zip(n_in, n_out, factor)]
+    anchor = _per_axis(anchor, dim)
+    axes = [LatticeAxis(a, f, n, m) for a, f, n, m in zip(anchor, factor, n_in, n_out)]
+    return n_in, n_out, factor, anchor, axes
 
 
 def resize(image, factor=None, shape=None, affine=None, anchor="c", dim=None, *args, **kwargs):
     """Resize an image by a factor or to a shape (nitorch/spatial/_grid.py:618-754).
 
-    anchor per dimension: 'c' (corner centres), 'e' (outer edges), 'f'
-    (first voxel), 'l' (last voxel); extra arguments go to grid_pull, with
-    prefilter=True by default.  Returns the resized image and, with
-    `affine`, its orientation matrix."""
+    The image is sampled by `grid_pull` (prefilter=True unless given; other
+    arguments go to it) on the lattice whose axes `LatticeAxis` describes:
+    anchor 'c' (voxel centres), 'e' (outer edges), 'f' (first voxel) or 'l'
+    (last voxel), per dimension.  Returns the resized image and, when
+    `affine` is given, the resized lattice's orientation matrix."""
     from .affine import affine_resize
     image = torch.as_tensor(image)
-    nb_dim = dim or (len(shape) if shape is not None else image.dim() - 2)
-    inshape = image.shape[-nb_dim:]
-    info = {"dtype": image.dtype, "device": image.device}
-    if torch.is_tensor(factor):
-        factor = factor.tolist()
-    factor = _pad_last(factor if isinstance(factor, (list, tuple)) else [factor or 0.], nb_dim)
-    factor = [float(f or 0.) for f in factor]
-    outshape = _pad_last(shape if shape is not None else [None], nb_dim)
-    anchor = [a[0].lower() for a in _pad_last(anchor if isinstance(anchor, (list, tuple)) else [anchor],
-                                               nb_dim)]
-    return_trf = kwargs.pop("_return_trf", False)
-
-    outshape = [o or max(1, int(i * f)) for i, o, f in zip(inshape, outshape, factor)]
-    if any(not s for s in outshape):
-        raise ValueError("Either factor or shape must be set in all dimensions")
-    factor = [f or o / i for o, i, f in zip(outshape, inshape, factor)]
-
-    # input coordinate = scale * output coordinate + shift, per dimension
-    lin, scales, shifts = [], [], []
-    for anch, f, inshp, outshp in zip(anchor, factor, inshape, outshape):
-        if inshp == 1 or outshp == 1:
-            anch = "e"
-        if anch == "c":
-            lin.append(torch.linspace(0, inshp - 1, outshp, **info))
-            scales.append((inshp - 1) / (outshp - 1))
-            shifts.append(0)
-        elif anch == "e":
-            scale = inshp / outshp
-            shift = 0.5 * (scale - 1)
-            lin.append(torch.arange(0., outshp, **info) * scale + shift)
-            scales.append(scale)
-            shifts.append(shift)
-        elif anch == "f":
-            lin.append(torch.arange(0., outshp, **info) / f)
-            scales.append(1 / f)
-            shifts.append(0)
-        elif anch == "l":
-            shift = (inshp - 1) - (outshp - 1) / f
-            lin.append(torch.arange(0., outshp, **info) / f + shift)
-            scales.append(1 / f)
-            shifts.append(shift)
-        else:
-            raise ValueError("Unknown anchor {}".format(anch))
-        if inshp == outshp == 1:
-            scales[-1] = 1 / f
-    grid = torch.stack(torch.meshgrid(*lin, indexing="ij"), dim=-1)
-
+    with_map = kwargs.pop("_return_trf", False)
+    n_in, n_out, factor, anchor, axes = _resize_lattice(image, factor, shape, anchor, dim)
+    lines = [ax.coordinates(dtype=image.dtype, device=image.device) for ax in axes]
+    lattice = torch.stack(torch.meshgrid(*lines, indexing="ij"), dim=-1)
     kwargs.setdefault("prefilter", True)
-    resized = grid_pull(image, grid, *args, **kwargs)
-
+    result = [grid_pull(image, lattice, *args, **kwargs)]
     if affine is not None:
-        affine, _ = affine_resize(affine, inshape, factor, anchor, shape_out=outshape)
-        if return_trf:
-            return resized, affine, (scales, shifts)
-        return resized, affine
-    if return_trf:
-        return resized, (scales, shifts)
-    return resized
+        result.append(affine_resize(affine, n_in, factor, anchor, shape_out=n_out)[0])
+    if with_map:
+        result.append(([ax.scale for ax in axes], [ax.shift for ax in axes]))
+    return result[0] if len(result) == 1 else tuple(result)
 
 
 def resize_grid(grid, factor=None, shape=None, type="grid", affine=None, *args, **kwargs):
     """Resize a transformation (type 'grid') or displacement field and
-    express it in the resized voxel space (nitorch/spatial/_grid.py:757-858)."""
+    express it in the resized voxel space (nitorch/spatial/_grid.py:757-858).
+
+    Each component is resampled like an image, then mapped from input to
+    output voxel units by the inverse of the axis map `scale * o + shift`:
+    (v - shift) / scale for coordinates, v / scale for displacements."""
     kwargs["_return_trf"] = True
     kwargs.setdefault("prefilter", True)
     ndim = grid.shape[-1]
-    spatial = grid.shape[-ndim - 1:-1]
-    batch = grid.shape[:-ndim - 1]
-    if len(batch) == 0:
-        grid = grid[None]
-    elif len(batch) > 1:
-        grid = grid.reshape([-1, *spatial, ndim])
+    lead = tuple(grid.shape[:grid.dim() - ndim - 1])
+    fields = grid.reshape((-1,) + tuple(grid.shape[grid.dim() - ndim - 1:])).movedim(-1, 1)
     if affine is not None:
-        affine = affine.expand([*batch, *affine.shape[-2:]])
-        affine = affine.reshape([-1, *affine.shape[-2:]])
-    grid = grid.movedim(-1, 1)
-    outputs = resize(grid, factor, shape, affine, *args, **kwargs)
-    if affine is not None:
-        grid, affine, (scales, shifts) = outputs
+        affine = affine.expand(lead + tuple(affine.shape[-2:])).reshape((-1,) + tuple(affine.shape[-2:]))
+        fields, affine, (scales, shifts) = resize(fields, factor, shape, affine, *args, **kwargs)
     else:
-        grid, (scales, shifts) = outputs
-    grid = grid.movedim(1, -1)
-    # original = scale * resized + shift  ->  resized = (original - shift) / scale
-    comps = []
-    for d, (scl, shft) in enumerate(zip(scales, shifts)):
-        g1 = grid[..., d]
-        if type[0].lower() == "g":
-            g1 = g1 - shft
-        comps.append(g1 / scl)
-    grid = torch.stack(comps, -1)
-    grid = grid.reshape([*batch, *grid.shape[-ndim - 1:]])
+        fields, (scales, shifts) = resize(fields, factor, shape, affine, *args, **kwargs)
+    is_coord = type[0].lower() == "g"
+    parts = [(fields[:, d] - shifts[d] if is_coord else fields[:, d]) / scales[d] for d in range(ndim)]
+    out = torch.stack(parts, -1)
+    out = out.reshape(lead + tuple(out.shape[1:]))
     if affine is not None:
-        affine = affine.reshape([*batch, *affine.shape[-2:]])
-        return grid, affine
-    return grid
+        return out, affine.reshape(lead + tuple(affine.shape[-2:]))
+    return out
 
 
 def reslice(image, affine, affine_to, shape_to=None, **kwargs):
     """Resample an image into another space (shape + affine)
-    (nitorch/spatial/_grid.py:861-915); grid_pull options, prefilter=True
-    by default."""
+    (nitorch/spatial/_grid.py:861-915): the target voxel v is read at the
+    source voxel inv(affine) @ affine_to @ v, by `grid_pull` with its
+    options (prefilter=True unless given).  Image (*batch, *channels,
+    *spatial) with batch the affine's leading dimensions; returns
+    (*batch, *channels, *shape_to)."""
     from .affine import affine_lmdiv
     kwargs.setdefault("prefilter", True)
     image = torch.as_tensor(image)
-    backend = dict(dtype=image.dtype, device=image.device)
-    if not backend["dtype"].is_floating_point:
-        backend["dtype"] = torch.get_default_dtype()
-    affine = torch.as_tensor(affine).to(**backend)
-    affine_to = torch.as_tensor(affine_to).to(**backend)
+    dtype = image.dtype if image.dtype.is_floating_point else torch.get_default_dtype()
+    affine = torch.as_tensor(affine).to(dtype=dtype, device=image.device)
+    affine_to = torch.as_tensor(affine_to).to(dtype=dtype, device=image.device)
     dim = affine.shape[-1] - 1
-    batch = affine.shape[:-2]
-    channels = image.shape[len(batch):-dim]
-    shape = image.shape[-dim:]
-    if shape_to is None:
-        shape_to = shape
-
-    def prod(s):
-        p = 1
-        for v in s:
-            p *= v
-        return p
-
-    transformation = affine_lmdiv(affine, affine_to)
-    grid = affine_grid(transformation, shape_to)
-    if not prod(batch):
-        grid = grid[None]
-    image = image.reshape([prod(batch) or 1, prod(channels) or 1, *shape])
-    image = grid_pull(image, grid, **kwargs)
-    return image.reshape([*batch, *channels, *shape_to])
+    lead = tuple(affine.shape[:-2])
+    chans = tuple(image.shape[len(lead):image.dim() - dim])
+    space = tuple(image.shape[image.dim() - dim:])
+    shape_to = tuple(space if shape_to is None else shape_to)
+    nbatch = int(torch.Size(lead).numel()) if lead else 0
+    nchan = int(torch.Size(chans).numel()) if chans else 0
+    lattice = affine_grid(affine_lmdiv(affine, affine_to), shape_to)
+    if not nbatch:
+        lattice = lattice[None]
+    out = grid_pull(image.reshape((nbatch or 1, nchan or 1) + space), lattice, **kwargs)
+    return out.reshape(lead + chans + shape_to)
 
 
 def _diff_central(x, dims, bound="dct2", voxel_size=1):
