@@ -167,27 +167,30 @@ int ni_resize(int32_t dtype, int32_t dim, int64_t N, int64_t C, const int64_t *i
  * Displacement-field solver (3-D).  Fields are [N, 3, *shape] and Hessians
  * [N, CC, *shape], contiguous, with CC in {1, 3, 6} (6 = h00 h11 h22 h01 h02
  * h12) or hessian == NULL.  `penalty` = {absolute, membrane, bending,
- * lame_shear, lame_div}; `voxel_size` has 3 entries.
+ * lame_shear, lame_div}; `voxel_size` has 3 entries.  `weight` (NULL for
+ * none): voxel-wise RLS weights [N, 1, *shape], contiguous, of the absolute
+ * and membrane energies (the reference's `weight` argument; bending and Lame
+ * penalties with weights are an error, as there).
  * ---------------------------------------------------------------------- */
 
 /* Replaces ni::regulariser_grid (nitorch/_C/src/regulariser_grid.h:8-11):
  * output = (H + L) input. */
 int ni_regulariser_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *input, void *output,
-                        const void *hessian, int32_t CC, const int32_t *bound, const double *voxel_size,
-                        const double *penalty, void *stream);
+                        const void *hessian, int32_t CC, const void *weight, const int32_t *bound,
+                        const double *voxel_size, const double *penalty, void *stream);
 
 /* Residual g - (H + L) x (the reference's residuals_grid,
  * nitorch/_C/src/impl/fmg_common.cpp:334-352).  If `norms` (device, 2 doubles)
  * is given it receives ||g - (H+L)x||^2 and ||g||^2. */
 int ni_residual_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *solution, const void *gradient,
-                     void *residual, const void *hessian, int32_t CC, const int32_t *bound,
+                     void *residual, const void *hessian, int32_t CC, const void *weight, const int32_t *bound,
                      const double *voxel_size, const double *penalty, double *norms, void *stream);
 
 /* Replaces ni::relax_grid (nitorch/_C/src/relax_grid.h:8-13): nb_iter
  * multi-colour Gauss-Seidel sweeps on (H + L) x = g, x updated in place. */
 int ni_relax_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
-                  const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
-                  const double *penalty, int32_t nb_iter, void *stream);
+                  const void *weight, const void *gradient, void *solution, const int32_t *bound,
+                  const double *voxel_size, const double *penalty, int32_t nb_iter, void *stream);
 
 /* Replace ni::fmg_restriction / ni::fmg_prolongation (nitorch/_C/src/multires.h):
  * [N, C, *in_shape] -> [N, C, *out_shape] (linear-adjoint restriction,
@@ -198,9 +201,9 @@ int ni_fmg_prolongation(int32_t dtype, int64_t N, int64_t C, const int64_t *in_s
                         const void *input, void *output, const int32_t *bound, void *stream);
 
 /* Bytes of device workspace ni_fmg_grid needs for this problem (use_cg:
- * with the PCG smoother's scratch). */
+ * with the PCG smoother's scratch; has_weight: with the RLS weight pyramid). */
 int64_t ni_fmg_grid_workspace(int32_t dtype, int64_t N, const int64_t *shape, int32_t CC, int32_t max_levels,
-                              int32_t use_cg);
+                              int32_t use_cg, int32_t has_weight);
 
 /* Replaces ni::fmg_grid (nitorch/_C/src/fmg.h:22-37): full multigrid with
  * nb_cycles V-cycles per level and nb_iter smoothing steps (use_cg = 0:
@@ -208,9 +211,10 @@ int64_t ni_fmg_grid_workspace(int32_t dtype, int64_t N, const int64_t *shape, in
  * solver='cg'), then `extra_vcycles` further V-cycles on the finest level.
  * `solution` is the initial guess (zeros for none) and receives the result. */
 int ni_fmg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
-                const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
-                const double *penalty, int32_t nb_cycles, int32_t nb_iter, int32_t max_levels,
-                int32_t use_cg, int32_t extra_vcycles, void *workspace, int64_t workspace_bytes, void *stream);
+                const void *weight, const void *gradient, void *solution, const int32_t *bound,
+                const double *voxel_size, const double *penalty, int32_t nb_cycles, int32_t nb_iter,
+                int32_t max_levels, int32_t use_cg, int32_t extra_vcycles, void *workspace, int64_t workspace_bytes,
+                void *stream);
 
 /* Time-to-tolerance solve (no reference counterpart: the benchmark's "GN step
  * to 1e-4 relative residual"): ni_fmg_grid's full multigrid (skipped when
@@ -220,9 +224,9 @@ int ni_fmg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hess
  * per V-cycle to read the residual.  Reports the final relative residual
  * and the number of extra V-cycles. */
 int ni_fmg_solve(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
-                 const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
-                 const double *penalty, int32_t nb_cycles, int32_t nb_iter, int32_t max_levels,
-                 int32_t use_cg, double tol, int32_t max_vcycles, void *workspace,
+                 const void *weight, const void *gradient, void *solution, const int32_t *bound,
+                 const double *voxel_size, const double *penalty, int32_t nb_cycles, int32_t nb_iter,
+                 int32_t max_levels, int32_t use_cg, double tol, int32_t max_vcycles, void *workspace,
                  int64_t workspace_bytes, double *out_rel, int32_t *out_vcycles, void *stream);
 
 /* Bytes of device workspace ni_pcg_grid needs (r, z, p and the partial sums
@@ -237,16 +241,18 @@ int64_t ni_pcg_grid_workspace(int32_t dtype, int64_t N, const int64_t *shape);
  * sum_n alpha_n (alpha_n p'Ap + 2 r'z) < tol * N * prod(shape).
  * out_iters (optional, host) receives the iterations done. */
 int ni_pcg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
-                const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
-                const double *penalty, int32_t nb_iter, double tol, void *workspace, int64_t workspace_bytes,
-                int32_t *out_iters, void *stream);
+                const void *weight, const void *gradient, void *solution, const int32_t *bound,
+                const double *voxel_size, const double *penalty, int32_t nb_iter, double tol, void *workspace,
+                int64_t workspace_bytes, int32_t *out_iters, void *stream);
 
 /* Replaces ni::precond_grid (nitorch/_C/src/precond_grid.h:8-12,
  * precond_grid_common.cpp:266-298, :620-660):
- * output = (H (1 + 1e-6) + diag(L) (1 + 1e-6))^-1 gradient, voxel by voxel. */
+ * output = (H (1 + 1e-6) + diag(L) (1 + 1e-6))^-1 gradient, voxel by voxel
+ * (with RLS weights, the weighted operator's diagonal, which reads the
+ * neighbours' weights through `bound`). */
 int ni_precond_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
-                    const void *gradient, void *output, const int32_t *bound, const double *voxel_size,
-                    const double *penalty, void *stream);
+                    const void *weight, const void *gradient, void *output, const int32_t *bound,
+                    const double *voxel_size, const double *penalty, void *stream);
 
 /* ------------------------------------------------------------------------
  * Learned preconditioner (3-D U-Net on BF16 matrix-core kernels).  No

@@ -9,6 +9,9 @@
 //                     invert3d_* :829-976)
 //   fmg transfers     multires_common.cpp (resize3d :430-499, restrict3d :586-669)
 //   fmg_grid          fmg_common.cpp (:45-222, :356-533)
+//   RLS weights       regulariser_grid_common.cpp:1171-1248 (vel2mom3d_rls_*),
+//                     relax_grid_common.cpp:1415-1500, precond_grid_common.cpp:
+//                     800-840, fmg_common.cpp:176-181 (absolute / membrane only)
 // with two deliberate departures from the reference (documented in the
 // project notes): relaxation visits every colour class (the reference's fold
 // start indices skip some, relax_grid_common.cpp:469-470), and the FMG
@@ -61,6 +64,7 @@ __device__ __forceinline__ long long sv_block() {
 struct StencilW {
   double vx[3];        // 1 / voxel_size^2
   double lam0;         // absolute
+  double lam1;         // membrane
   double w110, w101, w011, w200, w020, w002;
   double wd[3][4];     // per component: centre, x, y, z (Lame + scaled membrane/bending)
   double w2;           // Lame cross weight
@@ -77,6 +81,7 @@ static StencilW make_weights(const double *vs, const double *pen) {
                         l2 * (-4.0 * v1 * (v0 + v1 + v2)) - l1 * v1,
                         l2 * (-4.0 * v2 * (v0 + v1 + v2)) - l1 * v2};
   W.lam0 = l0;
+  W.lam1 = l1;
   W.w200 = l2 * v0 * v0; W.w020 = l2 * v1 * v1; W.w002 = l2 * v2 * v2;
   W.w110 = l2 * 2.0 * v0 * v1; W.w101 = l2 * 2.0 * v0 * v2; W.w011 = l2 * 2.0 * v1 * v2;
   const double v[3] = {v0, v1, v2};
@@ -95,11 +100,18 @@ struct StencilT {
   T lam0, w110, w101, w011, w200, w020, w002, w2;
   T wd[3][4];
   T diag[3];           // relax diagonal: wd[d][0] * (1 + 1e-6)
+  // voxel-wise RLS weights [N, 1, X, Y, Z] of the absolute / membrane
+  // energies (null: none), and the membrane weight of each axis
+  // (-membrane / voxel_size^2) that a face weight multiplies
+  const T *wmap;
+  T mw[3];
 };
 
 template <typename T>
-static StencilT<T> cast_weights(const StencilW &W) {
+static StencilT<T> cast_weights(const StencilW &W, const T *wmap = nullptr) {
   StencilT<T> S;
+  S.wmap = wmap;
+  for (int d = 0; d < 3; ++d) S.mw[d] = T(-W.lam1 * W.vx[d]);
   for (int d = 0; d < 3; ++d) {
     S.ivx[d] = T(1.0 / W.vx[d]);
     for (int k = 0; k < 4; ++k) S.wd[d][k] = T(W.wd[d][k]);
@@ -211,6 +223,81 @@ __device__ __forceinline__ void apply_L(const T *__restrict__ u, int V, const Ge
     apply_L_t<T, false>(u, V, G, S, x, y, z, out);
 }
 
+// Voxel-wise weighted (RLS) operator at (x, y, z): each of the six
+// first-order faces carries  mw[axis] * (w_c + w_nb)  (the neighbour's
+// weight read with the bound's index and sign, as a field value), and for
+// component d
+//   (L_w u)_d = (absolute w_c u_c + 1/2 sum_faces fw (u_nb - u_c)) voxel_size_d^2
+//   diag_d    = (absolute w_c     - 1/2 sum_faces fw              ) voxel_size_d^2
+// wn: weights of this batch element; u: component 0 of its field.
+template <typename T>
+__device__ __forceinline__ void rls_faces(const T *__restrict__ wn, const Geo &G, const StencilT<T> &S, int x,
+                                          int y, int z, int off[6], int sg[6], T fw[6], T &wc, T &fsum) {
+  const int sYZ = G.Y * G.Z, sZ = G.Z;
+  const int i = x * sYZ + y * sZ + z;
+  off[0] = fold<int>(G.b[0], x - 1, G.X, sg[0]) * sYZ + y * sZ + z;
+  off[1] = fold<int>(G.b[0], x + 1, G.X, sg[1]) * sYZ + y * sZ + z;
+  off[2] = x * sYZ + fold<int>(G.b[1], y - 1, G.Y, sg[2]) * sZ + z;
+  off[3] = x * sYZ + fold<int>(G.b[1], y + 1, G.Y, sg[3]) * sZ + z;
+  off[4] = x * sYZ + y * sZ + fold<int>(G.b[2], z - 1, G.Z, sg[4]);
+  off[5] = x * sYZ + y * sZ + fold<int>(G.b[2], z + 1, G.Z, sg[5]);
+  wc = wn[i];
+  fsum = T(0);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const T wv = sg[k] == 0 ? T(0) : (sg[k] < 0 ? -wn[off[k]] : wn[off[k]]);
+    fw[k] = S.mw[k >> 1] * (wc + wv);
+    fsum += fw[k];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void rls_diag(const T *__restrict__ wn, const Geo &G, const StencilT<T> &S, int x, int y,
+                                         int z, T dg[3]) {
+  int off[6], sg[6];
+  T fw[6], wc, fs;
+  rls_faces<T>(wn, G, S, x, y, z, off, sg, fw, wc, fs);
+  const T dw = S.lam0 * wc - T(0.5) * fs;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) dg[d] = dw * S.ivx[d];
+}
+
+template <typename T>
+__device__ __forceinline__ void apply_Lw(const T *__restrict__ u, const T *__restrict__ wn, int V, const Geo &G,
+                                         const StencilT<T> &S, int x, int y, int z, T out[3], T dg[3]) {
+  int off[6], sg[6];
+  T fw[6], wc, fs;
+  rls_faces<T>(wn, G, S, x, y, z, off, sg, fw, wc, fs);
+  const int i = (x * G.Y + y) * G.Z + z;
+  const T dw = S.lam0 * wc - T(0.5) * fs;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const T *ud = u + d * V;
+    const T c = ud[i];
+    T acc = T(0);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const T v = sg[k] == 0 ? T(0) : (sg[k] < 0 ? -ud[off[k]] : ud[off[k]]);
+      acc += fw[k] * (v - c);
+    }
+    out[d] = (S.lam0 * wc * c + T(0.5) * acc) * S.ivx[d];
+    dg[d] = dw * S.ivx[d];
+  }
+}
+
+// (L u) of batch element n at (x, y, z), with the diagonal used by the
+// relaxation and the preconditioner (with weights: the weighted operator's).
+template <typename T>
+__device__ __forceinline__ void apply_op(const T *__restrict__ u, int n, int V, const Geo &G, const StencilT<T> &S,
+                                         int x, int y, int z, T out[3], T dg[3]) {
+  if (S.wmap) {
+    apply_Lw<T>(u, S.wmap + (size_t)n * V, V, G, S, x, y, z, out, dg);
+  } else {
+    apply_L<T>(u, V, G, S, x, y, z, out);
+    dg[0] = S.diag[0]; dg[1] = S.diag[1]; dg[2] = S.diag[2];
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void load_H(const T *__restrict__ h, int CC, int V, int i, T M[6]) {
   M[0] = M[1] = M[2] = M[3] = M[4] = M[5] = T(0);
@@ -237,8 +324,8 @@ __global__ __launch_bounds__(SV_BLOCK) void reg_kernel(const T *__restrict__ x, 
     const int n = (int)(gid / V), i = (int)(gid % V);
     const int z = i % G.Z, y = (i / G.Z) % G.Y, xx = i / (G.Y * G.Z);
     const T *xn = x + (size_t)n * 3 * V;
-    T Lx[3];
-    apply_L<T>(xn, V, G, S, xx, y, z, Lx);
+    T Lx[3], dg[3];
+    apply_op<T>(xn, n, V, G, S, xx, y, z, Lx, dg);
     if (h && CC) {
       T M[6];
       load_H<T>(h + (size_t)n * CC * V, CC, V, i, M);
@@ -285,11 +372,11 @@ __global__ __launch_bounds__(SV_BLOCK) void reg_kernel(const T *__restrict__ x, 
 // One Gauss-Seidel update of voxel (xx, y, z) of batch n.
 template <typename T>
 __device__ __forceinline__ void relax_voxel(T *__restrict__ xn, const T *__restrict__ gn, const T *__restrict__ hn,
-                                            int CC, int V, const Geo &G, const StencilT<T> &S,
+                                            int CC, int V, const Geo &G, const StencilT<T> &S, int n,
                                             int xx, int y, int z) {
   const int i = (xx * G.Y + y) * G.Z + z;
-  T Lx[3];
-  apply_L<T>(xn, V, G, S, xx, y, z, Lx);
+  T Lx[3], dg[3];
+  apply_op<T>(xn, n, V, G, S, xx, y, z, Lx, dg);
   T r0 = gn[i] - Lx[0], r1 = gn[V + i] - Lx[1], r2 = gn[2 * V + i] - Lx[2];
   T M[6];
   load_H<T>(hn, hn ? CC : 0, V, i, M);
@@ -298,7 +385,7 @@ __device__ __forceinline__ void relax_voxel(T *__restrict__ xn, const T *__restr
   r1 -= M[3] * x0 + M[1] * x1 + M[5] * x2;
   r2 -= M[4] * x0 + M[5] * x1 + M[2] * x2;
   const T opt = T(1.000001);
-  const T a = M[0] * opt + S.diag[0], b = M[1] * opt + S.diag[1], c = M[2] * opt + S.diag[2];
+  const T a = M[0] * opt + dg[0], b = M[1] * opt + dg[1], c = M[2] * opt + dg[2];
   const T p = M[3], q = M[4], s = M[5];
   // 3x3 symmetric solve by the adjugate (relax_grid_common.cpp:840-852)
   const T idt = T(1) / (a * b * c - a * s * s - b * q * q - c * p * p + T(2) * p * q * s);
@@ -354,7 +441,7 @@ __global__ __launch_bounds__(SV_BLOCK) void relax_colour(T *__restrict__ x, cons
     if (z >= G.Z) return;
   }
   relax_voxel<T>(x + (size_t)n * 3 * V, g + (size_t)n * 3 * V, h ? h + (size_t)n * CC * V : nullptr,
-                 CC, V, G, S, xx, y, z);
+                 CC, V, G, S, n, xx, y, z);
 }
 
 // All sweeps of a small level in one work-group, with a barrier between
@@ -387,7 +474,7 @@ __global__ __launch_bounds__(relax_small_threads<T>()) void relax_small(T *__res
           const int z = (j - row * nzh) * 2 + ((xx + y + rb) & 1);
           if (z < G.Z)
             relax_voxel<T>(xs + (size_t)n * 3 * V, g + (size_t)n * 3 * V, h ? h + (size_t)n * CC * V : nullptr,
-                           CC, V, G, S, xx, y, z);
+                           CC, V, G, S, n, xx, y, z);
         }
         __syncthreads();
       }
@@ -405,7 +492,7 @@ __global__ __launch_bounds__(relax_small_threads<T>()) void relax_small(T *__res
             const int y = P.ax[1].start[b] + P.ax[1].step[b] * jy;
             const int xx = P.ax[0].start[a] + P.ax[0].step[a] * jx;
             relax_voxel<T>(xs + (size_t)n * 3 * V, g + (size_t)n * 3 * V, h ? h + (size_t)n * CC * V : nullptr,
-                           CC, V, G, S, xx, y, z);
+                           CC, V, G, S, n, xx, y, z);
           }
           __syncthreads();
         }
@@ -423,17 +510,28 @@ __global__ __launch_bounds__(relax_small_threads<T>()) void relax_small(T *__res
 // synchronisation.
 // --------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ void block_jacobi(const T *__restrict__ hn, int CC, int V, int i, const StencilT<T> &S,
+__device__ __forceinline__ void block_jacobi(const T *__restrict__ hn, int CC, int V, int i, const T dg[3],
                                              T v0, T v1, T v2, T &s0, T &s1, T &s2) {
   T M[6];
   load_H<T>(hn, hn ? CC : 0, V, i, M);
   const T opt = T(1.000001);
-  const T a = M[0] * opt + S.diag[0], b = M[1] * opt + S.diag[1], c = M[2] * opt + S.diag[2];
+  const T a = M[0] * opt + dg[0], b = M[1] * opt + dg[1], c = M[2] * opt + dg[2];
   const T p = M[3], q = M[4], s = M[5];
   const T idt = T(1) / (a * b * c - a * s * s - b * q * q - c * p * p + T(2) * p * q * s);
   s0 = idt * (v0 * (b * c - s * s) + v1 * (q * s - p * c) + v2 * (p * s - q * b));
   s1 = idt * (v0 * (q * s - p * c) + v1 * (a * c - q * q) + v2 * (p * q - a * s));
   s2 = idt * (v0 * (p * s - q * b) + v1 * (p * q - a * s) + v2 * (a * b - p * p));
+}
+
+// The preconditioner diagonal at voxel i of batch element n.
+template <typename T>
+__device__ __forceinline__ void voxel_diag(const Geo &G, const StencilT<T> &S, int n, int i, T dg[3]) {
+  if (S.wmap) {
+    const int z = i % G.Z, y = (i / G.Z) % G.Y, x = i / (G.Y * G.Z);
+    rls_diag<T>(S.wmap + (size_t)n * G.X * G.Y * G.Z, G, S, x, y, z, dg);
+  } else {
+    dg[0] = S.diag[0]; dg[1] = S.diag[1]; dg[2] = S.diag[2];
+  }
 }
 
 // Per-batch double reductions.  Each scalar is held as SV_SLOTS partial
@@ -518,8 +616,8 @@ __global__ __launch_bounds__(SV_BLOCK) void pcg_init(const T *__restrict__ x, co
     const int i = (int)(gid % V);
     const int zz = i % G.Z, y = (i / G.Z) % G.Y, xx = i / (G.Y * G.Z);
     const size_t b3 = (size_t)n * 3 * V;
-    T Lx[3];
-    apply_L<T>(x + b3, V, G, S, xx, y, zz, Lx);
+    T Lx[3], dg[3];
+    apply_op<T>(x + b3, n, V, G, S, xx, y, zz, Lx, dg);
     const T *hn = h ? h + (size_t)n * CC * V : nullptr;
     T M[6];
     load_H<T>(hn, hn ? CC : 0, V, i, M);
@@ -528,7 +626,7 @@ __global__ __launch_bounds__(SV_BLOCK) void pcg_init(const T *__restrict__ x, co
     const T r1 = g[b3 + V + i] - (Lx[1] + M[3] * x0 + M[1] * x1 + M[5] * x2);
     const T r2 = g[b3 + 2 * V + i] - (Lx[2] + M[4] * x0 + M[5] * x1 + M[2] * x2);
     T z0, z1, z2;
-    block_jacobi<T>(hn, CC, V, i, S, r0, r1, r2, z0, z1, z2);
+    block_jacobi<T>(hn, CC, V, i, dg, r0, r1, r2, z0, z1, z2);
     r[b3 + i] = r0; r[b3 + V + i] = r1; r[b3 + 2 * V + i] = r2;
     z[b3 + i] = z0; z[b3 + V + i] = z1; z[b3 + 2 * V + i] = z2;
     p[b3 + i] = z0; p[b3 + V + i] = z1; p[b3 + 2 * V + i] = z2;
@@ -551,8 +649,8 @@ __global__ __launch_bounds__(SV_BLOCK) void pcg_matvec(const T *__restrict__ p, 
     const int i = (int)(gid % V);
     const int zz = i % G.Z, y = (i / G.Z) % G.Y, xx = i / (G.Y * G.Z);
     const size_t b3 = (size_t)n * 3 * V;
-    T Lp[3];
-    apply_L<T>(p + b3, V, G, S, xx, y, zz, Lp);
+    T Lp[3], dg[3];
+    apply_op<T>(p + b3, n, V, G, S, xx, y, zz, Lp, dg);
     T M[6];
     load_H<T>(h ? h + (size_t)n * CC * V : nullptr, h ? CC : 0, V, i, M);
     const T p0 = p[b3 + i], p1 = p[b3 + V + i], p2 = p[b3 + 2 * V + i];
@@ -691,8 +789,9 @@ __global__ __launch_bounds__(SV_BLOCK) void pcg_update(T *__restrict__ x, T *__r
       rr[d] = r[k] - alpha * z[k];
       r[k] = rr[d];
     }
-    T z0, z1, z2;
-    block_jacobi<T>(h ? h + (size_t)n * CC * V : nullptr, CC, V, i, S, rr[0], rr[1], rr[2], z0, z1, z2);
+    T z0, z1, z2, dg[3];
+    voxel_diag<T>(G, S, n, i, dg);
+    block_jacobi<T>(h ? h + (size_t)n * CC * V : nullptr, CC, V, i, dg, rr[0], rr[1], rr[2], z0, z1, z2);
     z[b3 + i] = z0; z[b3 + V + i] = z1; z[b3 + 2 * V + i] = z2;
     acc = (double)rr[0] * z0 + (double)rr[1] * z1 + (double)rr[2] * z2;
   }
@@ -724,13 +823,15 @@ __global__ __launch_bounds__(SV_BLOCK) void pcg_direction(T *__restrict__ p, con
 // z = M \ g  (precond_grid)
 template <typename T>
 __global__ __launch_bounds__(SV_BLOCK) void precond_kernel(const T *__restrict__ g, const T *__restrict__ h, int CC,
-                                                           T *__restrict__ out, int V, int N, StencilT<T> S) {
+                                                           T *__restrict__ out, Geo G, int N, StencilT<T> S) {
+  const int V = G.X * G.Y * G.Z;
   const long long gid = sv_block() * SV_BLOCK + threadIdx.x;
   if (gid >= (long long)N * V) return;
   const int n = (int)(gid / V), i = (int)(gid % V);
   const size_t b3 = (size_t)n * 3 * V;
-  T s0, s1, s2;
-  block_jacobi<T>(h ? h + (size_t)n * CC * V : nullptr, CC, V, i, S, g[b3 + i], g[b3 + V + i], g[b3 + 2 * V + i],
+  T s0, s1, s2, dg[3];
+  voxel_diag<T>(G, S, n, i, dg);
+  block_jacobi<T>(h ? h + (size_t)n * CC * V : nullptr, CC, V, i, dg, g[b3 + i], g[b3 + V + i], g[b3 + 2 * V + i],
                   s0, s1, s2);
   out[b3 + i] = s0; out[b3 + V + i] = s1; out[b3 + 2 * V + i] = s2;
 }
@@ -960,6 +1061,10 @@ static size_t align_up(size_t b, size_t a) { return (b + a - 1) / a * a; }
 template <typename T>
 static void launch_matvec(const T *p, const T *h, int CC, T *Ap, const Geo &G, const StencilT<T> &S, int N,
                           double *pap, unsigned bv, hipStream_t st) {
+  if (S.wmap) {   // weighted operator: the per-voxel kernel
+    hipLaunchKernelGGL(pcg_matvec<T>, dim3(bv), dim3(SV_BLOCK), 0, st, p, h, CC, Ap, G, S, N, pap);
+    return;
+  }
   if constexpr (std::is_same<T, float>::value) {
     const int ntx = (G.X + MV_TX - 1) / MV_TX, nty = (G.Y + MV_TY - 1) / MV_TY, ntz = (G.Z + MV_TZ - 1) / MV_TZ;
     const long long nb = (long long)N * ntx * nty * ntz;
@@ -1034,20 +1139,21 @@ static int build_pyramid(const int *sz0, const double *vs0, int max_levels, Leve
   return n;
 }
 
-// workspace elements: level 0 residual (3V) + per coarse level h (CC V), g, x, r (3V each)
-static size_t fmg_ws_elems(int N, int CC, const Level *L, int NL) {
+// workspace elements: level 0 residual (3V) + per coarse level h (CC V),
+// RLS weights (V, when given), g, x, r (3V each)
+static size_t fmg_ws_elems(int N, int CC, int has_w, const Level *L, int NL) {
   size_t e = (size_t)N * 3 * L[0].V;
-  for (int l = 1; l < NL; ++l) e += (size_t)N * (CC + 9) * L[l].V;
+  for (int l = 1; l < NL; ++l) e += (size_t)N * (CC + (has_w ? 1 : 0) + 9) * L[l].V;
   return e;
 }
 
 // bytes: pyramid, the slots of the residual norms (2 SV_SLOTS doubles), then
 // (PCG smoother) z and p at level-0 size and 4 N SV_SLOTS doubles
-static size_t fmg_norms_offset(int N, int CC, const Level *L, int NL, size_t esz) {
-  return align_up(fmg_ws_elems(N, CC, L, NL) * esz, 256);
+static size_t fmg_norms_offset(int N, int CC, int has_w, const Level *L, int NL, size_t esz) {
+  return align_up(fmg_ws_elems(N, CC, has_w, L, NL) * esz, 256);
 }
-static size_t fmg_ws_bytes(int N, int CC, const Level *L, int NL, int use_cg, size_t esz) {
-  size_t b = fmg_norms_offset(N, CC, L, NL, esz) + 2 * SV_SLOTS * sizeof(double);
+static size_t fmg_ws_bytes(int N, int CC, int has_w, const Level *L, int NL, int use_cg, size_t esz) {
+  size_t b = fmg_norms_offset(N, CC, has_w, L, NL, esz) + 2 * SV_SLOTS * sizeof(double);
   if (use_cg)
     b = align_up(b + 2 * pcg_field_elems(N, L[0].V) * esz, 256) + 4 * (size_t)N * SV_SLOTS * sizeof(double);
   return b;
@@ -1058,6 +1164,7 @@ struct FMG {
   int N, CC, NL, band, nb_iter, use_cg;
   Level L[16];
   const T *h[16];
+  const T *w[16];          // RLS weights per level (null: none)
   const T *g0;
   T *g[16], *x[16], *r[16];
   StencilT<T> S[16];
@@ -1121,17 +1228,18 @@ struct FMG {
 };
 
 template <typename T>
-static int fmg_setup(FMG<T> &F, int N, const int *sz, const T *hess, int CC, const T *grad, T *sol,
-                     const int *bound, const double *vs, const double *pen, int nb_iter,
+static int fmg_setup(FMG<T> &F, int N, const int *sz, const T *hess, int CC, const T *weight, const T *grad,
+                     T *sol, const int *bound, const double *vs, const double *pen, int nb_iter,
                      int max_levels, int use_cg, void *ws, size_t ws_bytes, hipStream_t st) {
   F.N = N; F.CC = hess ? CC : 0; F.nb_iter = nb_iter; F.use_cg = use_cg; F.st = st;
   for (int d = 0; d < 3; ++d) F.bound[d] = bound[d];
   F.NL = build_pyramid(sz, vs, max_levels, F.L);
   F.band = relax_band(pen);
-  const size_t need = fmg_ws_bytes(N, F.CC, F.L, F.NL, use_cg, sizeof(T));
+  const int has_w = weight != nullptr;
+  const size_t need = fmg_ws_bytes(N, F.CC, has_w, F.L, F.NL, use_cg, sizeof(T));
   NI_CHECK(ws_bytes >= need, "fmg_grid: workspace too small (%zu < %zu bytes)", ws_bytes, need);
   T *p = (T *)ws;
-  const size_t noff = fmg_norms_offset(N, F.CC, F.L, F.NL, sizeof(T));
+  const size_t noff = fmg_norms_offset(N, F.CC, has_w, F.L, F.NL, sizeof(T));
   F.norm_slots = (double *)((char *)ws + noff);
   F.cg_z = F.cg_p = nullptr;
   F.cg_sc = nullptr;
@@ -1143,24 +1251,28 @@ static int fmg_setup(FMG<T> &F, int N, const int *sz, const T *hess, int CC, con
   }
   F.g0 = grad;
   F.h[0] = hess;
+  F.w[0] = weight;
   F.x[0] = sol;
   F.g[0] = nullptr;
   F.r[0] = p; p += (size_t)N * 3 * F.L[0].V;
   for (int l = 1; l < F.NL; ++l) {
     const size_t V = F.L[l].V;
     if (F.CC) { F.h[l] = p; p += (size_t)N * F.CC * V; } else { F.h[l] = nullptr; }
+    if (has_w) { F.w[l] = p; p += (size_t)N * V; } else { F.w[l] = nullptr; }
     F.g[l] = p; p += (size_t)N * 3 * V;
     F.x[l] = p; p += (size_t)N * 3 * V;
     F.r[l] = p; p += (size_t)N * 3 * V;
   }
   for (int l = 0; l < F.NL; ++l) {
-    F.S[l] = cast_weights<T>(make_weights(F.L[l].vs, pen));
+    F.S[l] = cast_weights<T>(make_weights(F.L[l].vs, pen), F.w[l]);
     F.G[l].X = F.L[l].sz[0]; F.G[l].Y = F.L[l].sz[1]; F.G[l].Z = F.L[l].sz[2];
     for (int d = 0; d < 3; ++d) F.G[l].b[d] = bound[d];
   }
   // pyramid of Hessians, gradients and initial solutions
   for (int l = 1; l < F.NL; ++l) {
     if (F.CC) F.restrict_hess(F.h[l - 1], (T *)F.h[l], l - 1);
+    // RLS weights: restricted like a field, never rescaled (fmg_common.cpp:409-414)
+    if (has_w) launch_transfer<T>(F.w[l - 1], (T *)F.w[l], N, 1, F.L[l - 1].sz, F.L[l].sz, bound, true, nullptr, 0, st);
     F.restrict_field(l == 1 ? grad : F.g[l - 1], F.g[l], l - 1);
     F.restrict_field(F.x[l - 1], F.x[l], l - 1);
   }
@@ -1175,6 +1287,14 @@ static void fmg_run(FMG<T> &F, int nb_cycles) {
     F.prolong(F.x[base + 1], F.x[base], base, 0);
     for (int c = 0; c < nb_cycles; ++c) F.vcycle(base);
   }
+}
+
+// RLS weights exist for the absolute and membrane energies only (the
+// reference raises for bending / Lame, regulariser_grid_common.cpp:304-306).
+static int rls_check(const void *weight, const double *pen) {
+  NI_CHECK(!weight || (pen[2] == 0 && pen[3] == 0 && pen[4] == 0),
+           "RLS weights are only implemented for the absolute and membrane energies");
+  return 0;
 }
 
 static int fmg_check(int N, const int *sz, int CC, const int *bound, int nb_iter) {
@@ -1195,12 +1315,13 @@ using namespace ni_amd;
 extern "C" {
 
 int ni_regulariser_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *input, void *output,
-                        const void *hessian, int32_t CC, const int32_t *bound, const double *voxel_size,
-                        const double *penalty, void *stream) {
+                        const void *hessian, int32_t CC, const void *weight, const int32_t *bound,
+                        const double *voxel_size, const double *penalty, void *stream) {
   set_error("");
   NI_CHECK(input && output && shape && bound && voxel_size && penalty, "regulariser_grid: missing argument");
   const int sz[3] = {(int)shape[0], (int)shape[1], (int)shape[2]};
   if (int rc = fmg_check((int)N, sz, hessian ? CC : 0, bound, 0)) return rc;
+  if (int rc = rls_check(weight, penalty)) return rc;
   Geo G;
   G.X = sz[0]; G.Y = sz[1]; G.Z = sz[2];
   for (int d = 0; d < 3; ++d) G.b[d] = bound[d];
@@ -1208,22 +1329,23 @@ int ni_regulariser_grid(int32_t dtype, int64_t N, const int64_t *shape, const vo
   hipStream_t st = (hipStream_t)stream;
   if (dtype == NI_FLOAT32)
     launch_reg<float>((const float *)input, (const float *)hessian, hessian ? CC : 0, nullptr, (float *)output, G,
-                      cast_weights<float>(W), (int)N, 0, nullptr, st);
+                      cast_weights<float>(W, (const float *)weight), (int)N, 0, nullptr, st);
   else if (dtype == NI_FLOAT64)
     launch_reg<double>((const double *)input, (const double *)hessian, hessian ? CC : 0, nullptr, (double *)output, G,
-                       cast_weights<double>(W), (int)N, 0, nullptr, st);
+                       cast_weights<double>(W, (const double *)weight), (int)N, 0, nullptr, st);
   else NI_CHECK(false, "regulariser_grid: unsupported dtype %d", dtype);
   NI_HIP_CHECK(hipGetLastError());
   return 0;
 }
 
 int ni_residual_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *solution, const void *gradient,
-                     void *residual, const void *hessian, int32_t CC, const int32_t *bound,
+                     void *residual, const void *hessian, int32_t CC, const void *weight, const int32_t *bound,
                      const double *voxel_size, const double *penalty, double *norms, void *stream) {
   set_error("");
   NI_CHECK(solution && gradient && residual && shape && bound && voxel_size && penalty, "residual_grid: missing argument");
   const int sz[3] = {(int)shape[0], (int)shape[1], (int)shape[2]};
   if (int rc = fmg_check((int)N, sz, hessian ? CC : 0, bound, 0)) return rc;
+  if (int rc = rls_check(weight, penalty)) return rc;
   Geo G;
   G.X = sz[0]; G.Y = sz[1]; G.Z = sz[2];
   for (int d = 0; d < 3; ++d) G.b[d] = bound[d];
@@ -1232,22 +1354,23 @@ int ni_residual_grid(int32_t dtype, int64_t N, const int64_t *shape, const void 
   if (norms) launch_fill<double>(norms, 2, 0.0, st);
   if (dtype == NI_FLOAT32)
     launch_reg<float>((const float *)solution, (const float *)hessian, hessian ? CC : 0, (const float *)gradient,
-                      (float *)residual, G, cast_weights<float>(W), (int)N, 1, norms, st);
+                      (float *)residual, G, cast_weights<float>(W, (const float *)weight), (int)N, 1, norms, st);
   else if (dtype == NI_FLOAT64)
     launch_reg<double>((const double *)solution, (const double *)hessian, hessian ? CC : 0, (const double *)gradient,
-                       (double *)residual, G, cast_weights<double>(W), (int)N, 1, norms, st);
+                       (double *)residual, G, cast_weights<double>(W, (const double *)weight), (int)N, 1, norms, st);
   else NI_CHECK(false, "residual_grid: unsupported dtype %d", dtype);
   NI_HIP_CHECK(hipGetLastError());
   return 0;
 }
 
 int ni_relax_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
-                  const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
-                  const double *penalty, int32_t nb_iter, void *stream) {
+                  const void *weight, const void *gradient, void *solution, const int32_t *bound,
+                  const double *voxel_size, const double *penalty, int32_t nb_iter, void *stream) {
   set_error("");
   NI_CHECK(gradient && solution && shape && bound && voxel_size && penalty, "relax_grid: missing argument");
   const int sz[3] = {(int)shape[0], (int)shape[1], (int)shape[2]};
   if (int rc = fmg_check((int)N, sz, hessian ? CC : 0, bound, nb_iter)) return rc;
+  if (int rc = rls_check(weight, penalty)) return rc;
   Geo G;
   G.X = sz[0]; G.Y = sz[1]; G.Z = sz[2];
   for (int d = 0; d < 3; ++d) G.b[d] = bound[d];
@@ -1256,10 +1379,10 @@ int ni_relax_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *he
   hipStream_t st = (hipStream_t)stream;
   if (dtype == NI_FLOAT32)
     launch_relax<float>((float *)solution, (const float *)gradient, (const float *)hessian, hessian ? CC : 0, G,
-                        cast_weights<float>(W), (int)N, band, nb_iter, st);
+                        cast_weights<float>(W, (const float *)weight), (int)N, band, nb_iter, st);
   else if (dtype == NI_FLOAT64)
     launch_relax<double>((double *)solution, (const double *)gradient, (const double *)hessian, hessian ? CC : 0, G,
-                         cast_weights<double>(W), (int)N, band, nb_iter, st);
+                         cast_weights<double>(W, (const double *)weight), (int)N, band, nb_iter, st);
   else NI_CHECK(false, "relax_grid: unsupported dtype %d", dtype);
   NI_HIP_CHECK(hipGetLastError());
   return 0;
@@ -1296,35 +1419,38 @@ int ni_fmg_prolongation(int32_t dtype, int64_t N, int64_t C, const int64_t *in_s
 }
 
 int64_t ni_fmg_grid_workspace(int32_t dtype, int64_t N, const int64_t *shape, int32_t CC, int32_t max_levels,
-                              int32_t use_cg) {
+                              int32_t use_cg, int32_t has_weight) {
   Level L[16];
   const int sz[3] = {(int)shape[0], (int)shape[1], (int)shape[2]};
   const double vs[3] = {1, 1, 1};
   const int NL = build_pyramid(sz, vs, max_levels < 1 ? 1 : max_levels, L);
-  return (int64_t)fmg_ws_bytes((int)N, CC, L, NL, use_cg != 0, dtype == NI_FLOAT64 ? 8 : 4);
+  return (int64_t)fmg_ws_bytes((int)N, CC, has_weight != 0, L, NL, use_cg != 0, dtype == NI_FLOAT64 ? 8 : 4);
 }
 
 int ni_fmg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
-                const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
-                const double *penalty, int32_t nb_cycles, int32_t nb_iter, int32_t max_levels,
-                int32_t use_cg, int32_t extra_vcycles, void *workspace, int64_t workspace_bytes, void *stream) {
+                const void *weight, const void *gradient, void *solution, const int32_t *bound,
+                const double *voxel_size, const double *penalty, int32_t nb_cycles, int32_t nb_iter,
+                int32_t max_levels, int32_t use_cg, int32_t extra_vcycles, void *workspace, int64_t workspace_bytes,
+                void *stream) {
   set_error("");
   NI_CHECK(gradient && solution && shape && bound && voxel_size && penalty && workspace, "fmg_grid: missing argument");
   const int sz[3] = {(int)shape[0], (int)shape[1], (int)shape[2]};
   if (int rc = fmg_check((int)N, sz, hessian ? CC : 0, bound, nb_iter)) return rc;
+  if (int rc = rls_check(weight, penalty)) return rc;
   hipStream_t st = (hipStream_t)stream;
   if (dtype == NI_FLOAT32) {
     FMG<float> F;
-    if (int rc = fmg_setup<float>(F, (int)N, sz, (const float *)hessian, CC, (const float *)gradient,
+    if (int rc = fmg_setup<float>(F, (int)N, sz, (const float *)hessian, CC, (const float *)weight, (const float *)gradient,
                                   (float *)solution, bound, voxel_size, penalty, nb_iter, max_levels < 1 ? 1 : max_levels,
                                   use_cg != 0, workspace, (size_t)workspace_bytes, st)) return rc;
     fmg_run<float>(F, nb_cycles);
     for (int k = 0; k < extra_vcycles; ++k) F.vcycle(0);
   } else if (dtype == NI_FLOAT64) {
     FMG<double> F;
-    if (int rc = fmg_setup<double>(F, (int)N, sz, (const double *)hessian, CC, (const double *)gradient,
-                                   (double *)solution, bound, voxel_size, penalty, nb_iter, max_levels < 1 ? 1 : max_levels,
-                                   use_cg != 0, workspace, (size_t)workspace_bytes, st)) return rc;
+    if (int rc = fmg_setup<double>(F, (int)N, sz, (const double *)hessian, CC, (const double *)weight,
+                                   (const double *)gradient, (double *)solution, bound, voxel_size, penalty, nb_iter,
+                                   max_levels < 1 ? 1 : max_levels, use_cg != 0, workspace, (size_t)workspace_bytes,
+                                   st)) return rc;
     fmg_run<double>(F, nb_cycles);
     for (int k = 0; k < extra_vcycles; ++k) F.vcycle(0);
   } else {
@@ -1335,15 +1461,16 @@ int ni_fmg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hess
 }
 
 int ni_fmg_solve(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
-                 const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
-                 const double *penalty, int32_t nb_cycles, int32_t nb_iter, int32_t max_levels,
-                 int32_t use_cg, double tol, int32_t max_vcycles, void *workspace, int64_t workspace_bytes,
-                 double *out_rel, int32_t *out_vcycles, void *stream) {
+                 const void *weight, const void *gradient, void *solution, const int32_t *bound,
+                 const double *voxel_size, const double *penalty, int32_t nb_cycles, int32_t nb_iter,
+                 int32_t max_levels, int32_t use_cg, double tol, int32_t max_vcycles, void *workspace,
+                 int64_t workspace_bytes, double *out_rel, int32_t *out_vcycles, void *stream) {
   set_error("");
   NI_CHECK(gradient && solution && shape && bound && voxel_size && penalty && workspace,
            "fmg_solve: missing argument");
   const int sz[3] = {(int)shape[0], (int)shape[1], (int)shape[2]};
   if (int rc = fmg_check((int)N, sz, hessian ? CC : 0, bound, nb_iter)) return rc;
+  if (int rc = rls_check(weight, penalty)) return rc;
   hipStream_t st = (hipStream_t)stream;
   std::vector<double> host(2 * SV_SLOTS);
   int cycles = 0;
@@ -1367,15 +1494,17 @@ int ni_fmg_solve(int32_t dtype, int64_t N, const int64_t *shape, const void *hes
   };
   if (dtype == NI_FLOAT32) {
     FMG<float> F;
-    if (int rc = fmg_setup<float>(F, (int)N, sz, (const float *)hessian, CC, (const float *)gradient,
-                                  (float *)solution, bound, voxel_size, penalty, nb_iter, max_levels < 1 ? 1 : max_levels,
-                                  use_cg != 0, workspace, (size_t)workspace_bytes, st)) return rc;
+    if (int rc = fmg_setup<float>(F, (int)N, sz, (const float *)hessian, CC, (const float *)weight,
+                                  (const float *)gradient, (float *)solution, bound, voxel_size, penalty, nb_iter,
+                                  max_levels < 1 ? 1 : max_levels, use_cg != 0, workspace, (size_t)workspace_bytes,
+                                  st)) return rc;
     if (int rc = loop(F)) return rc;
   } else if (dtype == NI_FLOAT64) {
     FMG<double> F;
-    if (int rc = fmg_setup<double>(F, (int)N, sz, (const double *)hessian, CC, (const double *)gradient,
-                                   (double *)solution, bound, voxel_size, penalty, nb_iter, max_levels < 1 ? 1 : max_levels,
-                                   use_cg != 0, workspace, (size_t)workspace_bytes, st)) return rc;
+    if (int rc = fmg_setup<double>(F, (int)N, sz, (const double *)hessian, CC, (const double *)weight,
+                                   (const double *)gradient, (double *)solution, bound, voxel_size, penalty, nb_iter,
+                                   max_levels < 1 ? 1 : max_levels, use_cg != 0, workspace, (size_t)workspace_bytes,
+                                   st)) return rc;
     if (int rc = loop(F)) return rc;
   } else {
     NI_CHECK(false, "fmg_solve: unsupported dtype %d", dtype);
@@ -1393,13 +1522,14 @@ int64_t ni_pcg_grid_workspace(int32_t dtype, int64_t N, const int64_t *shape) {
 }
 
 int ni_pcg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
-                const void *gradient, void *solution, const int32_t *bound, const double *voxel_size,
-                const double *penalty, int32_t nb_iter, double tol, void *workspace, int64_t workspace_bytes,
-                int32_t *out_iters, void *stream) {
+                const void *weight, const void *gradient, void *solution, const int32_t *bound,
+                const double *voxel_size, const double *penalty, int32_t nb_iter, double tol, void *workspace,
+                int64_t workspace_bytes, int32_t *out_iters, void *stream) {
   set_error("");
   NI_CHECK(gradient && solution && shape && bound && voxel_size && penalty && workspace, "pcg_grid: missing argument");
   const int sz[3] = {(int)shape[0], (int)shape[1], (int)shape[2]};
   if (int rc = fmg_check((int)N, sz, hessian ? CC : 0, bound, nb_iter)) return rc;
+  if (int rc = rls_check(weight, penalty)) return rc;
   NI_CHECK(tol >= 0, "pcg_grid: tol must be >= 0");
   const int64_t need = ni_pcg_grid_workspace(dtype, N, shape);
   NI_CHECK(workspace_bytes >= need, "pcg_grid: workspace too small (%lld < %lld bytes)", (long long)workspace_bytes,
@@ -1416,7 +1546,7 @@ int ni_pcg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hess
     T *r = (T *)workspace, *z = r + fe, *p = z + fe;
     double *sc = (double *)((char *)workspace + align_up(3 * fe * sizeof(T), 256));
     return run_pcg<T>((T *)solution, (const T *)gradient, (const T *)hessian, hessian ? CC : 0, G,
-                      cast_weights<T>(W), (int)N, nb_iter, tol, r, z, p, sc, st, &iters);
+                      cast_weights<T>(W, (const T *)weight), (int)N, nb_iter, tol, r, z, p, sc, st, &iters);
   };
   int rc;
   if (dtype == NI_FLOAT32) rc = go((float *)nullptr);
@@ -1429,22 +1559,28 @@ int ni_pcg_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hess
 }
 
 int ni_precond_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *hessian, int32_t CC,
-                    const void *gradient, void *output, const int32_t *bound, const double *voxel_size,
-                    const double *penalty, void *stream) {
+                    const void *weight, const void *gradient, void *output, const int32_t *bound,
+                    const double *voxel_size, const double *penalty, void *stream) {
   set_error("");
   NI_CHECK(gradient && output && shape && bound && voxel_size && penalty, "precond_grid: missing argument");
   const int sz[3] = {(int)shape[0], (int)shape[1], (int)shape[2]};
   if (int rc = fmg_check((int)N, sz, hessian ? CC : 0, bound, 0)) return rc;
+  if (int rc = rls_check(weight, penalty)) return rc;
   const StencilW W = make_weights(voxel_size, penalty);
+  Geo G;
+  G.X = sz[0]; G.Y = sz[1]; G.Z = sz[2];
+  for (int d = 0; d < 3; ++d) G.b[d] = bound[d];
   const int V = sz[0] * sz[1] * sz[2];
   hipStream_t st = (hipStream_t)stream;
   const unsigned nb = nblocks((long long)N * V);
   if (dtype == NI_FLOAT32)
     hipLaunchKernelGGL(precond_kernel<float>, dim3(nb), dim3(SV_BLOCK), 0, st, (const float *)gradient,
-                       (const float *)hessian, hessian ? CC : 0, (float *)output, V, (int)N, cast_weights<float>(W));
+                       (const float *)hessian, hessian ? CC : 0, (float *)output, G, (int)N,
+                       cast_weights<float>(W, (const float *)weight));
   else if (dtype == NI_FLOAT64)
     hipLaunchKernelGGL(precond_kernel<double>, dim3(nb), dim3(SV_BLOCK), 0, st, (const double *)gradient,
-                       (const double *)hessian, hessian ? CC : 0, (double *)output, V, (int)N, cast_weights<double>(W));
+                       (const double *)hessian, hessian ? CC : 0, (double *)output, G, (int)N,
+                       cast_weights<double>(W, (const double *)weight));
   else NI_CHECK(false, "precond_grid: unsupported dtype %d", dtype);
   NI_HIP_CHECK(hipGetLastError());
   return 0;

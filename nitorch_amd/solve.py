@@ -8,7 +8,10 @@ libnitorch_amd.so.
 Shapes follow the reference's Python layer: fields `(N, *spatial, 3)`,
 Hessians `(N, *spatial, CC)` with CC in {1, 3, 6} (channel last); 3-D only.
 Penalties: absolute, membrane, bending, lame=(shear, div), all multiplied by
-`factor`.  Voxel-wise weights (RLS) are not provided.
+`factor`.  Voxel-wise weights (RLS) `weight` of shape (N, *spatial) (or
+(N, *spatial, 1)) weight the absolute and membrane energies, as the
+reference's `weight` argument (bending / Lame penalties with weights raise,
+as there).
 """
 import ctypes
 
@@ -18,25 +21,25 @@ from . import _lib as L
 from .grid import bound_to_nitorch, make_list
 
 L.register_signatures({
-    "ni_regulariser_grid": [L._I32, L._I64, L._PI64, L._P, L._P, L._P, L._I32, L._PI32,
+    "ni_regulariser_grid": [L._I32, L._I64, L._PI64, L._P, L._P, L._P, L._I32, L._P, L._PI32,
                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), L._P],
-    "ni_residual_grid": [L._I32, L._I64, L._PI64, L._P, L._P, L._P, L._P, L._I32, L._PI32,
+    "ni_residual_grid": [L._I32, L._I64, L._PI64, L._P, L._P, L._P, L._P, L._I32, L._P, L._PI32,
                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), L._P, L._P],
-    "ni_relax_grid": [L._I32, L._I64, L._PI64, L._P, L._I32, L._P, L._P, L._PI32,
+    "ni_relax_grid": [L._I32, L._I64, L._PI64, L._P, L._I32, L._P, L._P, L._P, L._PI32,
                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), L._I32, L._P],
     "ni_fmg_restriction": [L._I32, L._I64, L._I64, L._PI64, L._PI64, L._P, L._P, L._PI32, L._P],
     "ni_fmg_prolongation": [L._I32, L._I64, L._I64, L._PI64, L._PI64, L._P, L._P, L._PI32, L._P],
-    "ni_fmg_grid": [L._I32, L._I64, L._PI64, L._P, L._I32, L._P, L._P, L._PI32,
+    "ni_fmg_grid": [L._I32, L._I64, L._PI64, L._P, L._I32, L._P, L._P, L._P, L._PI32,
                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                     L._I32, L._I32, L._I32, L._I32, L._I32, L._P, L._I64, L._P],
-    "ni_fmg_solve": [L._I32, L._I64, L._PI64, L._P, L._I32, L._P, L._P, L._PI32,
+    "ni_fmg_solve": [L._I32, L._I64, L._PI64, L._P, L._I32, L._P, L._P, L._P, L._PI32,
                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                      L._I32, L._I32, L._I32, L._I32, ctypes.c_double, L._I32, L._P, L._I64,
                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32), L._P],
-    "ni_pcg_grid": [L._I32, L._I64, L._PI64, L._P, L._I32, L._P, L._P, L._PI32,
+    "ni_pcg_grid": [L._I32, L._I64, L._PI64, L._P, L._I32, L._P, L._P, L._P, L._PI32,
                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                     L._I32, ctypes.c_double, L._P, L._I64, ctypes.POINTER(ctypes.c_int32), L._P],
-    "ni_precond_grid": [L._I32, L._I64, L._PI64, L._P, L._I32, L._P, L._P, L._PI32,
+    "ni_precond_grid": [L._I32, L._I64, L._PI64, L._P, L._I32, L._P, L._P, L._P, L._PI32,
                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), L._P],
 })
 
@@ -44,7 +47,7 @@ L.register_signatures({
 def _ws_fn():
     fn = L.lib().ni_fmg_grid_workspace
     fn.restype = ctypes.c_int64
-    fn.argtypes = [L._I32, L._I64, L._PI64, L._I32, L._I32, L._I32]
+    fn.argtypes = [L._I32, L._I64, L._PI64, L._I32, L._I32, L._I32, L._I32]
     return fn
 
 
@@ -99,47 +102,59 @@ def _check(gradient, hessian):
             raise ValueError("hessian and gradient must share dtype and device")
 
 
-def _no_weight(weight):
-    if weight is not None:
-        raise NotImplementedError("voxel-wise (RLS) weights are not provided by the HIP solver")
+def _weight_cf(weight, like):
+    """RLS weights of the reference's grid layer -- (N, *spatial), or
+    (N, *spatial, 1) -- as a contiguous (N, 1, *spatial) tensor broadcast to
+    the batch of `like` (channel-first field)."""
+    if weight is None:
+        return None
+    w = torch.as_tensor(weight, dtype=like.dtype, device=like.device)
+    dim = like.dim() - 2
+    if w.dim() == dim + 2 and w.shape[-1] == 1:
+        w = w[..., 0]
+    while w.dim() < dim + 1:
+        w = w.unsqueeze(0)
+    w = w.unsqueeze(1)
+    return w.expand((like.shape[0], 1) + tuple(like.shape[2:])).contiguous()
 
 
 # ---------------------------------------------------------------------------
 # channel-first primitives (the reference's C++ layer)
 # ---------------------------------------------------------------------------
 def regulariser_grid_cf(x, hessian=None, penalty=(0, 0, 0, 0, 0), voxel_size=(1, 1, 1),
-                        bound="dft", out=None):
-    """out = (H + L) x for contiguous x [N, 3, X, Y, Z], H [N, CC, X, Y, Z]."""
+                        bound="dft", out=None, weight=None):
+    """out = (H + L) x for contiguous x [N, 3, X, Y, Z], H [N, CC, X, Y, Z],
+    RLS weights [N, 1, X, Y, Z]."""
     out = torch.empty_like(x) if out is None else out
     CC = 0 if hessian is None else hessian.shape[1]
     L.check(L.lib().ni_regulariser_grid(
         L.dtype_code(x.dtype), x.shape[0], L.i64arr(x.shape[2:]), L.ptr(x), L.ptr(out),
-        L.ptr(hessian), CC, _bounds(bound), _voxel(voxel_size), _dvec(penalty), L.stream_of(x)),
-        "regulariser_grid")
+        L.ptr(hessian), CC, L.ptr(weight), _bounds(bound), _voxel(voxel_size), _dvec(penalty),
+        L.stream_of(x)), "regulariser_grid")
     return out
 
 
 def residual_grid_cf(x, g, hessian=None, penalty=(0, 0, 0, 0, 0), voxel_size=(1, 1, 1),
-                     bound="dft", out=None, norms=None):
+                     bound="dft", out=None, norms=None, weight=None):
     """r = g - (H + L) x; if `norms` (float64 device tensor of 2) is given it
     receives ||r||^2 and ||g||^2."""
     out = torch.empty_like(x) if out is None else out
     CC = 0 if hessian is None else hessian.shape[1]
     L.check(L.lib().ni_residual_grid(
         L.dtype_code(x.dtype), x.shape[0], L.i64arr(x.shape[2:]), L.ptr(x), L.ptr(g), L.ptr(out),
-        L.ptr(hessian), CC, _bounds(bound), _voxel(voxel_size), _dvec(penalty), L.ptr(norms),
-        L.stream_of(x)), "residual_grid")
+        L.ptr(hessian), CC, L.ptr(weight), _bounds(bound), _voxel(voxel_size), _dvec(penalty),
+        L.ptr(norms), L.stream_of(x)), "residual_grid")
     return out
 
 
 def relax_grid_cf(g, x, hessian=None, penalty=(0, 0, 0, 0, 0), voxel_size=(1, 1, 1),
-                  bound="dft", nb_iter=2):
+                  bound="dft", nb_iter=2, weight=None):
     """nb_iter Gauss-Seidel sweeps, x updated in place."""
     CC = 0 if hessian is None else hessian.shape[1]
     L.check(L.lib().ni_relax_grid(
-        L.dtype_code(g.dtype), g.shape[0], L.i64arr(g.shape[2:]), L.ptr(hessian), CC, L.ptr(g),
-        L.ptr(x), _bounds(bound), _voxel(voxel_size), _dvec(penalty), int(nb_iter), L.stream_of(g)),
-        "relax_grid")
+        L.dtype_code(g.dtype), g.shape[0], L.i64arr(g.shape[2:]), L.ptr(hessian), CC, L.ptr(weight),
+        L.ptr(g), L.ptr(x), _bounds(bound), _voxel(voxel_size), _dvec(penalty), int(nb_iter),
+        L.stream_of(g)), "relax_grid")
     return x
 
 
@@ -171,17 +186,18 @@ def fmg_prolongation(input, output=None, bound="dct2"):
 
 def fmg_grid_cf(hessian, g, x=None, penalty=(0, 0, 0, 0, 0), voxel_size=(1, 1, 1), bound="dft",
                 nb_cycles=2, nb_iter=2, max_levels=16, extra_vcycles=0, workspace=None,
-                use_cg=False):
+                use_cg=False, weight=None):
     """Full multigrid on contiguous channel-first tensors; x updated in place.
     use_cg: PCG smoothing (the reference's solver='cg'), else Gauss-Seidel."""
     x = torch.zeros_like(g) if x is None else x
     CC = 0 if hessian is None else hessian.shape[1]
     dt = L.dtype_code(g.dtype)
-    nbytes = _ws_fn()(dt, g.shape[0], L.i64arr(g.shape[2:]), CC, int(max_levels), int(bool(use_cg)))
+    nbytes = _ws_fn()(dt, g.shape[0], L.i64arr(g.shape[2:]), CC, int(max_levels), int(bool(use_cg)),
+                      int(weight is not None))
     if workspace is None or workspace.numel() < nbytes:
         workspace = torch.empty(int(nbytes), dtype=torch.uint8, device=g.device)
     L.check(L.lib().ni_fmg_grid(
-        dt, g.shape[0], L.i64arr(g.shape[2:]), L.ptr(hessian), CC, L.ptr(g), L.ptr(x),
+        dt, g.shape[0], L.i64arr(g.shape[2:]), L.ptr(hessian), CC, L.ptr(weight), L.ptr(g), L.ptr(x),
         _bounds(bound), _voxel(voxel_size), _dvec(penalty), int(nb_cycles), int(nb_iter),
         int(max_levels), int(bool(use_cg)), int(extra_vcycles), L.ptr(workspace), int(nbytes),
         L.stream_of(g)), "fmg_grid")
@@ -189,7 +205,7 @@ def fmg_grid_cf(hessian, g, x=None, penalty=(0, 0, 0, 0, 0), voxel_size=(1, 1, 1
 
 
 def pcg_grid_cf(hessian, g, x=None, penalty=(0, 0, 0, 0, 0), voxel_size=(1, 1, 1), bound="dft",
-                nb_iter=16, tol=0.0, workspace=None):
+                nb_iter=16, tol=0.0, workspace=None, weight=None):
     """Block-Jacobi preconditioned CG (nitorch/_C/src/impl/pcg_common.cpp:38-85) on
     contiguous channel-first tensors; x updated in place.  Returns (x, iterations)."""
     x = torch.zeros_like(g) if x is None else x
@@ -200,18 +216,19 @@ def pcg_grid_cf(hessian, g, x=None, penalty=(0, 0, 0, 0, 0), voxel_size=(1, 1, 1
         workspace = torch.empty(int(nbytes), dtype=torch.uint8, device=g.device)
     iters = ctypes.c_int32(0)
     L.check(L.lib().ni_pcg_grid(
-        dt, g.shape[0], L.i64arr(g.shape[2:]), L.ptr(hessian), CC, L.ptr(g), L.ptr(x),
+        dt, g.shape[0], L.i64arr(g.shape[2:]), L.ptr(hessian), CC, L.ptr(weight), L.ptr(g), L.ptr(x),
         _bounds(bound), _voxel(voxel_size), _dvec(penalty), int(nb_iter), float(tol),
         L.ptr(workspace), int(nbytes), ctypes.byref(iters), L.stream_of(g)), "pcg_grid")
     return x, iters.value
 
 
-def precond_grid_cf(hessian, g, penalty=(0, 0, 0, 0, 0), voxel_size=(1, 1, 1), bound="dft", out=None):
+def precond_grid_cf(hessian, g, penalty=(0, 0, 0, 0, 0), voxel_size=(1, 1, 1), bound="dft", out=None,
+                    weight=None):
     """(H + diag L)^-1 g voxel by voxel (precond_grid_common.cpp:620-660)."""
     out = torch.empty_like(g) if out is None else out
     CC = 0 if hessian is None else hessian.shape[1]
     L.check(L.lib().ni_precond_grid(
-        L.dtype_code(g.dtype), g.shape[0], L.i64arr(g.shape[2:]), L.ptr(hessian), CC, L.ptr(g),
+        L.dtype_code(g.dtype), g.shape[0], L.i64arr(g.shape[2:]), L.ptr(hessian), CC, L.ptr(weight), L.ptr(g),
         L.ptr(out), _bounds(bound), _voxel(voxel_size), _dvec(penalty), L.stream_of(g)),
         "precond_grid")
     return out
@@ -224,13 +241,12 @@ def regulariser_grid(input, weight=None, hessian=None, absolute=0, membrane=0, b
                      lame=0, factor=1, voxel_size=1, bound="dft", output=None):
     """(hessian + regulariser) @ input for input (N, *shape, 3)
     (nitorch/_C/solve.py c_regulariser_grid)."""
-    _no_weight(weight)
     _check(input, hessian)
     pad = 5 - input.dim()
     x = _to_cf(input, "input")
     h = _to_cf(hessian, "hessian")
     out = regulariser_grid_cf(x, h, _penalty(absolute, membrane, bending, lame, factor),
-                              voxel_size, bound)
+                              voxel_size, bound, weight=_weight_cf(weight, x))
     out = out.movedim(1, -1)
     if output is not None:
         output.copy_(out.reshape(output.shape))
@@ -244,14 +260,13 @@ def relax_grid(gradient, weight=None, hessian=None, absolute=0, membrane=0, bend
                factor=1, voxel_size=1, bound="dft", nb_iter=2, output=None):
     """Gauss-Seidel relaxation of (H + L) x = g (nitorch/_C/solve.py c_relax_grid);
     `output` is the initial guess and receives the result."""
-    _no_weight(weight)
     _check(gradient, hessian)
     pad = 5 - gradient.dim()
     g = _to_cf(gradient, "gradient")
     h = _to_cf(hessian, "hessian")
     x = torch.zeros_like(g) if output is None else _to_cf(output, "output")
     relax_grid_cf(g, x, h, _penalty(absolute, membrane, bending, lame, factor), voxel_size,
-                  bound, nb_iter)
+                  bound, nb_iter, weight=_weight_cf(weight, g))
     res = x.movedim(1, -1)
     if output is not None:
         output.copy_(res.reshape(output.shape))
@@ -279,7 +294,6 @@ def fmg_grid(hessian, gradient, weight=None, absolute=0, membrane=0, bending=0, 
     solver: 'cg' (block-Jacobi PCG smoothing, the reference's default) or
     'relax' (multi-colour Gauss-Seidel).
     """
-    _no_weight(weight)
     if solver == "relaks":
         solver = "relax"
     if solver not in ("relax", "cg"):
@@ -290,7 +304,7 @@ def fmg_grid(hessian, gradient, weight=None, absolute=0, membrane=0, bending=0, 
     h = _to_cf(hessian, "hessian")
     x = torch.zeros_like(g) if output is None else _to_cf(output, "output")
     fmg_grid_cf(h, g, x, _penalty(absolute, membrane, bending, lame, factor), voxel_size,
-                bound, nb_cycles, nb_iter, max_levels, use_cg=(solver == "cg"))
+                bound, nb_cycles, nb_iter, max_levels, use_cg=(solver == "cg"), weight=_weight_cf(weight, g))
     return _finish(x, output, pad)
 
 
@@ -298,14 +312,13 @@ def pcg_grid(gradient, weight=None, hessian=None, absolute=0, membrane=0, bendin
              factor=1, voxel_size=1, bound="dft", nb_iter=16, tol=1e-4, output=None):
     """Preconditioned conjugate gradient on (H + L) x = g
     (nitorch/_C/solve.py c_pcg_grid); `output` is the initial guess."""
-    _no_weight(weight)
     _check(gradient, hessian)
     pad = 5 - gradient.dim()
     g = _to_cf(gradient, "gradient")
     h = _to_cf(hessian, "hessian")
     x = torch.zeros_like(g) if output is None else _to_cf(output, "output")
     pcg_grid_cf(h, g, x, _penalty(absolute, membrane, bending, lame, factor), voxel_size, bound,
-                nb_iter, tol)
+                nb_iter, tol, weight=_weight_cf(weight, g))
     return _finish(x, output, pad)
 
 
@@ -313,12 +326,12 @@ def precond_grid(gradient, weight=None, hessian=None, absolute=0, membrane=0, be
                  factor=1, voxel_size=1, bound="dft", output=None):
     """Block-Jacobi preconditioner (H + diag L)^-1 g
     (nitorch/_C/solve.py c_precond_grid)."""
-    _no_weight(weight)
     _check(gradient, hessian)
     pad = 5 - gradient.dim()
     g = _to_cf(gradient, "gradient")
     h = _to_cf(hessian, "hessian")
-    out = precond_grid_cf(h, g, _penalty(absolute, membrane, bending, lame, factor), voxel_size, bound)
+    out = precond_grid_cf(h, g, _penalty(absolute, membrane, bending, lame, factor), voxel_size, bound,
+                          weight=_weight_cf(weight, g))
     return _finish(out, output, pad)
 
 
@@ -326,12 +339,11 @@ def solve_grid_fmg(hessian, gradient, absolute=0, membrane=0, bending=0, lame=0,
                    voxel_size=1, bound="dft", weights=None, optim="cg", nb_cycles=2, nb_iter=2):
     """nitorch.spatial.solve_grid_fmg (nitorch/spatial/_solvers.py:978-1039) on the GPU.
     Without spatial penalties the system is voxel-wise and solved in closed form."""
-    _no_weight(weights)
     lam = make_list(lame) if lame is not None else [0]
-    if not (membrane or bending or any(lam)):
+    if not (membrane or bending or any(lam)) and weights is None:
         return solve_grid_closedform(hessian, gradient, absolute=absolute, factor=factor,
                                      voxel_size=voxel_size)
-    return fmg_grid(hessian, gradient, None, absolute, membrane, bending, lame, factor, voxel_size,
+    return fmg_grid(hessian, gradient, weights, absolute, membrane, bending, lame, factor, voxel_size,
                     bound, nb_cycles, nb_iter, solver=optim)
 
 
@@ -359,7 +371,7 @@ def solve_grid_closedform(hessian, gradient, absolute=0, factor=1, voxel_size=1)
 
 def solve_to_tolerance(hessian, gradient, penalty, voxel_size=(1, 1, 1), bound="dft",
                        tol=1e-4, nb_cycles=2, nb_iter=4, max_vcycles=20, max_levels=16, x=None,
-                       workspace=None, use_cg=False):
+                       workspace=None, use_cg=False, weight=None):
     """Plain GPU FMG followed by V-cycles on the finest level until
     ||g - (H + L) x|| / ||g|| <= tol.  Channel-first contiguous tensors.
     Returns (x, relative residual, number of extra V-cycles)."""
@@ -367,13 +379,14 @@ def solve_to_tolerance(hessian, gradient, penalty, voxel_size=(1, 1, 1), bound="
     x = torch.zeros_like(g) if x is None else x
     CC = 0 if h is None else h.shape[1]
     dt = L.dtype_code(g.dtype)
-    nbytes = _ws_fn()(dt, g.shape[0], L.i64arr(g.shape[2:]), CC, int(max_levels), int(bool(use_cg)))
+    nbytes = _ws_fn()(dt, g.shape[0], L.i64arr(g.shape[2:]), CC, int(max_levels), int(bool(use_cg)),
+                      int(weight is not None))
     if workspace is None or workspace.numel() < nbytes:
         workspace = torch.empty(int(nbytes), dtype=torch.uint8, device=g.device)
     rel = ctypes.c_double(0)
     ncyc = ctypes.c_int32(0)
     L.check(L.lib().ni_fmg_solve(
-        dt, g.shape[0], L.i64arr(g.shape[2:]), L.ptr(h), CC, L.ptr(g), L.ptr(x), _bounds(bound),
+        dt, g.shape[0], L.i64arr(g.shape[2:]), L.ptr(h), CC, L.ptr(weight), L.ptr(g), L.ptr(x), _bounds(bound),
         _voxel(voxel_size), _dvec(penalty), int(nb_cycles), int(nb_iter), int(max_levels),
         int(bool(use_cg)), float(tol), int(max_vcycles), L.ptr(workspace), int(nbytes),
         ctypes.byref(rel), ctypes.byref(ncyc), L.stream_of(g)), "fmg_solve")

@@ -2,7 +2,9 @@
 
 numpy front-end of `solver_oracle.c` (FP64) for 3-D displacement fields in
 the reference's C++ layout [N, 3, X, Y, Z]; Hessians [N, CC, X, Y, Z] with
-CC in {1, 3, 6} (6 = h00 h11 h22 h01 h02 h12).
+CC in {1, 3, 6} (6 = h00 h11 h22 h01 h02 h12).  `weight` (optional, every
+entry point): voxel-wise RLS weights [N, 1, X, Y, Z] (or [N, X, Y, Z]) of the
+absolute / membrane energies.
 """
 import ctypes
 
@@ -19,21 +21,21 @@ def _setup():
         return L
     D, I64, I = _c.POINTER(_c.c_double), _c.c_int64, _c.c_int
     PI, PI64 = _c.POINTER(_c.c_int), _c.POINTER(_c.c_int64)
-    L.or_regulariser_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5
+    L.or_regulariser_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5 + [D]
     L.or_regulariser_grid.restype = None
-    L.or_relax_grid.argtypes = [D, D, I, D, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5 + [I, I]
+    L.or_relax_grid.argtypes = [D, D, I, D, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5 + [I, I, D]
     L.or_relax_grid.restype = None
     L.or_prolong.argtypes = [D, D, I64, I64, PI64, PI64, PI]
     L.or_prolong.restype = None
     L.or_restrict.argtypes = [D, D, I64, I64, PI64, PI64, PI]
     L.or_restrict.restype = None
-    L.or_fmg_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5 + [I, I, I, I, I, I]
+    L.or_fmg_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5 + [I, I, I, I, I, I, D]
     L.or_fmg_grid.restype = None
-    L.or_cg_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5 + [I, _c.c_double]
+    L.or_cg_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5 + [I, _c.c_double, D]
     L.or_cg_grid.restype = I
-    L.or_pcg_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5 + [I, _c.c_double]
+    L.or_pcg_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5 + [I, _c.c_double, D]
     L.or_pcg_grid.restype = I
-    L.or_precond_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, D] + [_c.c_double] * 5
+    L.or_precond_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5 + [D]
     L.or_precond_grid.restype = None
     L._solver_ready = True
     return L
@@ -45,6 +47,16 @@ def _d(a):
 
 def _f64(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _w(weight, shape):
+    """RLS weights as a contiguous [N, 1, X, Y, Z] float64 array (or None)."""
+    if weight is None:
+        return None
+    w = np.asarray(weight, dtype=np.float64)
+    if w.ndim == 4:
+        w = w[:, None]
+    return np.ascontiguousarray(np.broadcast_to(w, (shape[0], 1) + tuple(shape[2:])))
 
 
 def _vs(voxel_size):
@@ -64,20 +76,21 @@ def _pen(absolute, membrane, bending, lame):
 
 
 def regulariser_grid(x, hessian=None, absolute=0, membrane=0, bending=0, lame=0,
-                     voxel_size=1, bound="dft"):
+                     voxel_size=1, bound="dft", weight=None):
     """(H + L) x for x [N, 3, X, Y, Z]."""
     L = _setup()
     x, h = _f64(x), _f64(hessian)
+    w = _w(weight, x.shape)
     out = np.empty_like(x)
     N, _, X, Y, Z = x.shape
     CC = 0 if h is None else h.shape[1]
     L.or_regulariser_grid(_d(x), _d(out), _d(h), CC, N, X, Y, Z, _b(bound), _vs(voxel_size),
-                          *_pen(absolute, membrane, bending, lame))
+                          *_pen(absolute, membrane, bending, lame), _d(w))
     return out
 
 
 def relax_grid(g, hessian=None, x=None, absolute=0, membrane=0, bending=0, lame=0,
-               voxel_size=1, bound="dft", nb_iter=1, quirk_fold=False, product_order=False):
+               voxel_size=1, bound="dft", nb_iter=1, quirk_fold=False, product_order=False, weight=None):
     """Gauss-Seidel sweeps.  quirk_fold: the reference's fold order and sign
     typos; product_order: the wrap-safe colour classes the HIP kernels use."""
     L = _setup()
@@ -86,8 +99,9 @@ def relax_grid(g, hessian=None, x=None, absolute=0, membrane=0, bending=0, lame=
     N, _, X, Y, Z = g.shape
     CC = 0 if h is None else h.shape[1]
     mode = 2 if product_order else int(bool(quirk_fold))
+    w = _w(weight, g.shape)
     L.or_relax_grid(_d(g), _d(h), CC, _d(x), N, X, Y, Z, _b(bound), _vs(voxel_size),
-                    *_pen(absolute, membrane, bending, lame), int(nb_iter), mode)
+                    *_pen(absolute, membrane, bending, lame), int(nb_iter), mode, _d(w))
     return x
 
 
@@ -113,51 +127,58 @@ def restrict(x, out_shape=None, bound="dct2"):
 
 def fmg_grid(hessian, g, x=None, absolute=0, membrane=0, bending=0, lame=0, voxel_size=1,
              bound="dft", nb_cycles=2, nb_iter=2, max_levels=16, quirk_fold=False, quirk_zscale=False,
-             product_order=False, use_cg=False):
+             product_order=False, use_cg=False, weight=None):
     L = _setup()
     g, h = _f64(g), _f64(hessian)
     x = np.zeros_like(g) if x is None else _f64(x).copy()
     N, _, X, Y, Z = g.shape
     CC = 0 if h is None else h.shape[1]
+    w = _w(weight, g.shape)
     L.or_fmg_grid(_d(h), _d(g), _d(x), CC, N, X, Y, Z, _b(bound), _vs(voxel_size),
                   *_pen(absolute, membrane, bending, lame), int(nb_cycles), int(nb_iter),
                   int(max_levels), 2 if product_order else int(bool(quirk_fold)), int(quirk_zscale),
-                  int(bool(use_cg)))
+                  int(bool(use_cg)), _d(w))
     return x
 
 
-def precond_grid(g, hessian=None, absolute=0, membrane=0, bending=0, lame=0, voxel_size=1):
-    """(H (1+1e-6) + diag(L) (1+1e-6))^-1 g, voxel by voxel."""
+def precond_grid(g, hessian=None, absolute=0, membrane=0, bending=0, lame=0, voxel_size=1,
+                 bound="dft", weight=None):
+    """(H (1+1e-6) + diag(L) (1+1e-6))^-1 g, voxel by voxel (with RLS
+    weights the diagonal of the weighted operator, which reads the
+    neighbours' weights through `bound`)."""
     L = _setup()
     g, h = _f64(g), _f64(hessian)
     out = np.empty_like(g)
     N, _, X, Y, Z = g.shape
     CC = 0 if h is None else h.shape[1]
-    L.or_precond_grid(_d(h), _d(g), _d(out), CC, N, X, Y, Z, _vs(voxel_size),
-                      *_pen(absolute, membrane, bending, lame))
+    w = _w(weight, g.shape)
+    L.or_precond_grid(_d(h), _d(g), _d(out), CC, N, X, Y, Z, _b(bound), _vs(voxel_size),
+                      *_pen(absolute, membrane, bending, lame), _d(w))
     return out
 
 
 def pcg_grid(hessian, g, x=None, absolute=0, membrane=0, bending=0, lame=0, voxel_size=1,
-             bound="dft", nb_iter=16, tol=0.0):
+             bound="dft", nb_iter=16, tol=0.0, weight=None):
     """Preconditioned CG as the reference's do_pcg; returns (x, iterations)."""
     L = _setup()
     g, h = _f64(g), _f64(hessian)
     x = np.zeros_like(g) if x is None else _f64(x).copy()
     N, _, X, Y, Z = g.shape
     CC = 0 if h is None else h.shape[1]
+    w = _w(weight, g.shape)
     it = L.or_pcg_grid(_d(h), _d(g), _d(x), CC, N, X, Y, Z, _b(bound), _vs(voxel_size),
-                       *_pen(absolute, membrane, bending, lame), int(nb_iter), float(tol))
+                       *_pen(absolute, membrane, bending, lame), int(nb_iter), float(tol), _d(w))
     return x, it
 
 
 def cg_grid(hessian, g, x=None, absolute=0, membrane=0, bending=0, lame=0, voxel_size=1,
-            bound="dft", max_iter=10000, tol=1e-12):
+            bound="dft", max_iter=10000, tol=1e-12, weight=None):
     L = _setup()
     g, h = _f64(g), _f64(hessian)
     x = np.zeros_like(g) if x is None else _f64(x).copy()
     N, _, X, Y, Z = g.shape
     CC = 0 if h is None else h.shape[1]
+    w = _w(weight, g.shape)
     it = L.or_cg_grid(_d(h), _d(g), _d(x), CC, N, X, Y, Z, _b(bound), _vs(voxel_size),
-                      *_pen(absolute, membrane, bending, lame), int(max_iter), float(tol))
+                      *_pen(absolute, membrane, bending, lame), int(max_iter), float(tol), _d(w))
     return x, it

@@ -18,6 +18,12 @@
  *   or_fmg_grid         : full multigrid (fmg_common.cpp:45-222, :356-533),
  *                         Gauss-Seidel or PCG smoother (:375-387)
  *   or_cg_grid          : plain CG on (H + L) x = g (exact solutions for tests)
+ * Every entry point takes an optional voxel-wise weight map w [N, 1, X, Y, Z]
+ * (RLS weights; absolute and membrane energies only, as the reference):
+ *   regulariser_grid_common.cpp:1171-1248 (vel2mom3d_rls_membrane/absolute),
+ *   relax_grid_common.cpp:1415-1500 (relax3d_rls_*),
+ *   precond_grid_common.cpp:800-840 (precond3d_rls_*),
+ *   fmg_common.cpp:176-181 (weights restricted like gradients, unscaled).
  *
  * Quirk switches reproduce the reference's own behaviour where it departs
  * from the intended algorithm, so that the restatement can be compared
@@ -198,10 +204,48 @@ static void hmat(const double *h, int CC, int64_t V, int64_t i, double M[3][3])
   }
 }
 
+/* RLS face weights at voxel (x, y, z) of the weight map wn: for the six
+ * first-order neighbours (x-1, x+1, y-1, y+1, z-1, z+1), the membrane weight
+ * of the axis (-membrane / voxel_size^2) times (w_centre + w_neighbour), the
+ * neighbour's weight read with the bound's index and sign like any field
+ * value.  Returns the centre weight. */
+static double rls_faces(const double *wn, const geom_t *G, const weights_t *W,
+                        int64_t x, int64_t y, int64_t z, double fw[6])
+{
+  double wc = wn[(x * G->Y + y) * G->Z + z];
+  const int64_t off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  for (int k = 0; k < 6; ++k) {
+    double m = -W->lam1 * W->vx[k / 2];
+    fw[k] = m * (wc + getv(wn, G, x + off[k][0], y + off[k][1], z + off[k][2]));
+  }
+  return wc;
+}
+
+/* ((L_w x)_d at (x, y, z), and the diagonal of L_w for component d):
+ *   (absolute w_c x_c + 1/2 sum_faces fw (x_nb - x_c)) / vx_d
+ *   (absolute w_c     - 1/2 sum_faces fw              ) / vx_d            */
+static double rls_component(const double *x3, const double *wn, const geom_t *G, const weights_t *W,
+                            int d, int64_t x, int64_t y, int64_t z, double *diag)
+{
+  int64_t V = G->X * G->Y * G->Z;
+  double fw[6];
+  double wc = rls_faces(wn, G, W, x, y, z, fw);
+  const double *xd = x3 + d * V;
+  double c = xd[(x * G->Y + y) * G->Z + z];
+  const int64_t off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  double acc = 0, fs = 0;
+  for (int k = 0; k < 6; ++k) {
+    acc += fw[k] * (getv(xd, G, x + off[k][0], y + off[k][1], z + off[k][2]) - c);
+    fs += fw[k];
+  }
+  if (diag) *diag = (W->lam0 * wc - 0.5 * fs) / W->vx[d];
+  return (W->lam0 * wc * c + 0.5 * acc) / W->vx[d];
+}
+
 void or_regulariser_grid(const double *x, double *out, const double *h, int CC,
                          int64_t N, int64_t X, int64_t Y, int64_t Z, const int *bound,
                          const double *voxel_size, double absolute, double membrane,
-                         double bending, double lame_shear, double lame_div)
+                         double bending, double lame_shear, double lame_div, const double *w)
 {
   weights_t W; set_weights(&W, voxel_size, absolute, membrane, bending, lame_shear, lame_div);
   geom_t G = {X, Y, Z, {bound[0], bound[1], bound[2]}};
@@ -209,13 +253,14 @@ void or_regulariser_grid(const double *x, double *out, const double *h, int CC,
   for (int64_t n = 0; n < N; ++n) {
     const double *xn = x + n * 3 * V;
     const double *hn = h ? h + n * CC * V : NULL;
+    const double *wn = w ? w + n * V : NULL;
     double *on = out + n * 3 * V;
     for (int64_t i = 0; i < V; ++i) {
       int64_t xx = i / (Y * Z), yy = (i / Z) % Y, zz = i % Z;
       double M[3][3];
       hmat(hn, hn ? CC : 0, V, i, M);
       for (int d = 0; d < 3; ++d) {
-        double v = lx_component(xn, &G, &W, d, xx, yy, zz);
+        double v = wn ? rls_component(xn, wn, &G, &W, d, xx, yy, zz, NULL) : lx_component(xn, &G, &W, d, xx, yy, zz);
         for (int e = 0; e < 3; ++e) v += M[d][e] * xn[e * V + i];
         on[d * V + i] = v;
       }
@@ -225,18 +270,26 @@ void or_regulariser_grid(const double *x, double *out, const double *h, int CC,
 
 /* One Gauss-Seidel update of voxel i (relax3d_all + invert3d_*). */
 static void relax_voxel(double *x3, const double *g3, const double *h, int CC, const geom_t *G,
-                        const weights_t *W, int64_t V, int64_t xx, int64_t yy, int64_t zz, int quirk)
+                        const weights_t *W, int64_t V, int64_t xx, int64_t yy, int64_t zz, int quirk,
+                        const double *wn)
 {
   int64_t i = (xx * G->Y + yy) * G->Z + zz;
-  double r[3], M[3][3];
-  for (int d = 0; d < 3; ++d) r[d] = g3[d * V + i] - lx_component_q(x3, G, W, d, xx, yy, zz, quirk);
+  double r[3], M[3][3], dg[3];
+  for (int d = 0; d < 3; ++d) {
+    if (wn) {
+      r[d] = g3[d * V + i] - rls_component(x3, wn, G, W, d, xx, yy, zz, &dg[d]);
+    } else {
+      r[d] = g3[d * V + i] - lx_component_q(x3, G, W, d, xx, yy, zz, quirk);
+      dg[d] = W->wd[d][0] * ONE_PLUS_TINY;
+    }
+  }
   hmat(h, h ? CC : 0, V, i, M);
   for (int d = 0; d < 3; ++d)
     for (int e = 0; e < 3; ++e) r[d] -= M[d][e] * x3[e * V + i];
   double A[3][3];
   for (int d = 0; d < 3; ++d)
     for (int e = 0; e < 3; ++e) A[d][e] = M[d][e] * (d == e ? ONE_PLUS_TINY : 1.0);
-  for (int d = 0; d < 3; ++d) A[d][d] += W->wd[d][0] * ONE_PLUS_TINY;
+  for (int d = 0; d < 3; ++d) A[d][d] += dg[d];
   /* 3x3 symmetric solve by the adjugate (relax_grid_common.cpp:840-852) */
   double a = A[0][0], b = A[1][1], c = A[2][2], p = A[0][1], q = A[0][2], s = A[1][2];
   double det = a * b * c - a * s * s - b * q * q - c * p * p + 2 * p * q * s;
@@ -273,7 +326,7 @@ void or_relax_grid(const double *g, const double *h, int CC, double *x,
                    int64_t N, int64_t X, int64_t Y, int64_t Z, const int *bound,
                    const double *voxel_size, double absolute, double membrane,
                    double bending, double lame_shear, double lame_div, int nb_iter,
-                   int quirk_fold)
+                   int quirk_fold, const double *w)
 {
   weights_t W; set_weights(&W, voxel_size, absolute, membrane, bending, lame_shear, lame_div);
   geom_t G = {X, Y, Z, {bound[0], bound[1], bound[2]}};
@@ -293,13 +346,14 @@ void or_relax_grid(const double *g, const double *h, int CC, double *x,
     double *xn = x + n * 3 * V;
     const double *gn = g + n * 3 * V;
     const double *hn = h ? h + n * CC * V : NULL;
+    const double *wn = w ? w + n * V : NULL;
     for (int it = 0; it < nb_iter; ++it) {
       if (bw == 0) {
         for (int rb = 0; rb < 2; ++rb)
           for (int64_t xx = 0; xx < X; ++xx)
             for (int64_t yy = 0; yy < Y; ++yy)
               for (int64_t zz = 0; zz < Z; ++zz)
-                if ((xx + yy + zz) % 2 == rb) relax_voxel(xn, gn, hn, CC, &G, &W, V, xx, yy, zz, quirk);
+                if ((xx + yy + zz) % 2 == rb) relax_voxel(xn, gn, hn, CC, &G, &W, V, xx, yy, zz, quirk, wn);
         continue;
       }
       if (safe) {
@@ -314,7 +368,7 @@ void or_relax_grid(const double *g, const double *h, int CC, double *x,
                 for (int64_t j = 0; j < cy.count[b]; ++j)
                   for (int64_t k = 0; k < cz.count[c]; ++k)
                     relax_voxel(xn, gn, hn, CC, &G, &W, V, cx.start[a] + cx.step[a] * i,
-                                cy.start[b] + cy.step[b] * j, cz.start[c] + cz.step[c] * k, 0);
+                                cy.start[b] + cy.step[b] * j, cz.start[c] + cz.step[c] * k, 0, wn);
         continue;
       }
       int64_t Fx = X < bw ? X : bw, Fy = Y < bw ? Y : bw, Fz = Z < bw ? Z : bw;
@@ -325,7 +379,7 @@ void or_relax_grid(const double *g, const double *h, int CC, double *x,
         for (int64_t xx = fx; xx < X; xx += Fx)
           for (int64_t yy = fy; yy < Y; yy += Fy)
             for (int64_t zz = fz; zz < Z; zz += Fz)
-              relax_voxel(xn, gn, hn, CC, &G, &W, V, xx, yy, zz, quirk);
+              relax_voxel(xn, gn, hn, CC, &G, &W, V, xx, yy, zz, quirk, wn);
       }
     }
   }
@@ -437,19 +491,30 @@ static void scale_fields(double *t, int64_t N, int64_t C, const int64_t *sz, con
  * the adjugate for CC = 6, divisions for CC in {0, 1, 3}
  * (precond_grid_common.cpp:620-700). */
 void or_precond_grid(const double *h, const double *g, double *out, int CC,
-                     int64_t N, int64_t X, int64_t Y, int64_t Z, const double *voxel_size,
-                     double absolute, double membrane, double bending, double lame_shear,
-                     double lame_div)
+                     int64_t N, int64_t X, int64_t Y, int64_t Z, const int *bound,
+                     const double *voxel_size, double absolute, double membrane, double bending,
+                     double lame_shear, double lame_div, const double *wmap)
 {
   weights_t W; set_weights(&W, voxel_size, absolute, membrane, bending, lame_shear, lame_div);
+  geom_t G = {X, Y, Z, {bound[0], bound[1], bound[2]}};
   int64_t V = X * Y * Z;
   double w[3];
-  for (int d = 0; d < 3; ++d) w[d] = W.wd[d][0] * ONE_PLUS_TINY;
   for (int64_t n = 0; n < N; ++n) {
     const double *hn = h ? h + n * CC * V : NULL;
     const double *gn = g + n * 3 * V;
+    const double *wn = wmap ? wmap + n * V : NULL;
     double *on = out + n * 3 * V;
     for (int64_t i = 0; i < V; ++i) {
+      for (int d = 0; d < 3; ++d) {
+        if (wn) {   /* RLS diagonal (no 1 + tiny on the regulariser part) */
+          int64_t xx = i / (Y * Z), yy = (i / Z) % Y, zz = i % Z;
+          double fw[6], fs = 0, wc = rls_faces(wn, &G, &W, xx, yy, zz, fw);
+          for (int k = 0; k < 6; ++k) fs += fw[k];
+          w[d] = (W.lam0 * wc - 0.5 * fs) / W.vx[d];
+        } else {
+          w[d] = W.wd[d][0] * ONE_PLUS_TINY;
+        }
+      }
       double v0 = gn[i], v1 = gn[V + i], v2 = gn[2 * V + i];
       if (!hn) {
         on[i] = v0 / w[0]; on[V + i] = v1 / w[1]; on[2 * V + i] = v2 / w[2];
@@ -487,7 +552,7 @@ static void batch_dot(const double *a, const double *b, int64_t N, int64_t n3, d
 int or_pcg_grid(const double *h, const double *g, double *x, int CC,
                 int64_t N, int64_t X, int64_t Y, int64_t Z, const int *bound,
                 const double *voxel_size, double absolute, double membrane, double bending,
-                double lame_shear, double lame_div, int nb_iter, double tol)
+                double lame_shear, double lame_div, int nb_iter, double tol, const double *w)
 {
   int64_t V = X * Y * Z, n3 = 3 * V, tot = N * n3;
   double *r = malloc(sizeof(double) * tot), *z = malloc(sizeof(double) * tot);
@@ -495,9 +560,9 @@ int or_pcg_grid(const double *h, const double *g, double *x, int CC,
   double *rz = malloc(sizeof(double) * N), *rz0 = malloc(sizeof(double) * N);
   double *pAp = malloc(sizeof(double) * N), *alpha = malloc(sizeof(double) * N);
 #define FWD(in, out) or_regulariser_grid(in, out, h, CC, N, X, Y, Z, bound, voxel_size, absolute, \
-                                         membrane, bending, lame_shear, lame_div)
-#define PRE(in, out) or_precond_grid(h, in, out, CC, N, X, Y, Z, voxel_size, absolute, membrane, \
-                                     bending, lame_shear, lame_div)
+                                         membrane, bending, lame_shear, lame_div, w)
+#define PRE(in, out) or_precond_grid(h, in, out, CC, N, X, Y, Z, bound, voxel_size, absolute, membrane, \
+                                     bending, lame_shear, lame_div, w)
   FWD(x, r);
   for (int64_t i = 0; i < tot; ++i) r[i] = g[i] - r[i];
   PRE(r, z);
@@ -536,11 +601,12 @@ void or_fmg_grid(const double *h, const double *g, double *x, int CC,
                  int64_t N, int64_t X, int64_t Y, int64_t Z, const int *bound,
                  const double *voxel_size, double absolute, double membrane, double bending,
                  double lame_shear, double lame_div, int nb_cycles, int nb_iter, int max_levels,
-                 int quirk_fold, int quirk_zscale, int use_cg)
+                 int quirk_fold, int quirk_zscale, int use_cg, const double *w)
 {
   int64_t sz[MAXL][3];
   double vs[MAXL][3];
-  double *H[MAXL], *Gr[MAXL], *Xs[MAXL], *R[MAXL];
+  double *H[MAXL], *Gr[MAXL], *Xs[MAXL], *R[MAXL], *Wm[MAXL];
+  Wm[0] = (double *)w;
   sz[0][0] = X; sz[0][1] = Y; sz[0][2] = Z;
   for (int d = 0; d < 3; ++d) vs[0][d] = voxel_size[d];
   int64_t V0 = X * Y * Z;
@@ -571,6 +637,10 @@ void or_fmg_grid(const double *h, const double *g, double *x, int CC,
     Gr[L] = (double *)malloc(sizeof(double) * N * 3 * V);
     or_restrict(Gr[L - 1], Gr[L], N, 3, sz[L - 1], sz[L], bound);
     scale_fields(Gr[L], N, 3, sz[L], f, 0, quirk_zscale);
+    if (w) {   /* weights: restricted, never rescaled (fmg_common.cpp:176-181, 409-414) */
+      Wm[L] = (double *)malloc(sizeof(double) * N * V);
+      or_restrict(Wm[L - 1], Wm[L], N, 1, sz[L - 1], sz[L], bound);
+    } else Wm[L] = NULL;
     R[L] = (double *)malloc(sizeof(double) * N * 3 * V);
     Xs[L] = (double *)malloc(sizeof(double) * N * 3 * V);
     or_restrict(Xs[L - 1], Xs[L], N, 3, sz[L - 1], sz[L], bound);
@@ -579,9 +649,9 @@ void or_fmg_grid(const double *h, const double *g, double *x, int CC,
   int NL = L;
 #define RELAX(l) do { \
     if (use_cg) or_pcg_grid(H[l], Gr[l], Xs[l], CC, N, sz[l][0], sz[l][1], sz[l][2], bound, vs[l], \
-                            absolute, membrane, bending, lame_shear, lame_div, nb_iter, 0.0); \
+                            absolute, membrane, bending, lame_shear, lame_div, nb_iter, 0.0, Wm[l]); \
     else or_relax_grid(Gr[l], H[l], CC, Xs[l], N, sz[l][0], sz[l][1], sz[l][2], bound, vs[l], \
-                       absolute, membrane, bending, lame_shear, lame_div, nb_iter, quirk_fold); \
+                       absolute, membrane, bending, lame_shear, lame_div, nb_iter, quirk_fold, Wm[l]); \
   } while (0)
 #define FAC(l, f) do { for (int d_ = 0; d_ < 3; ++d_) f[d_] = (double)sz[l][d_] / (double)sz[(l) + 1][d_]; } while (0)
   RELAX(NL - 1);
@@ -595,7 +665,7 @@ void or_fmg_grid(const double *h, const double *g, double *x, int CC,
         RELAX(l);
         int64_t V = sz[l][0] * sz[l][1] * sz[l][2];
         or_regulariser_grid(Xs[l], R[l], H[l], CC, N, sz[l][0], sz[l][1], sz[l][2], bound, vs[l],
-                            absolute, membrane, bending, lame_shear, lame_div);
+                            absolute, membrane, bending, lame_shear, lame_div, Wm[l]);
         for (int64_t i = 0; i < N * 3 * V; ++i) R[l][i] = Gr[l][i] - R[l][i];
         or_restrict(R[l], Gr[l + 1], N, 3, sz[l], sz[l + 1], bound);
         FAC(l, f);
@@ -617,7 +687,7 @@ void or_fmg_grid(const double *h, const double *g, double *x, int CC,
 #undef FAC
   for (int l = 0; l < NL; ++l) {
     free(Gr[l]); free(R[l]);
-    if (l > 0) { free(Xs[l]); if (H[l]) free(H[l]); }
+    if (l > 0) { free(Xs[l]); if (H[l]) free(H[l]); if (Wm[l]) free(Wm[l]); }
   }
 }
 
@@ -625,16 +695,16 @@ void or_fmg_grid(const double *h, const double *g, double *x, int CC,
 int or_cg_grid(const double *h, const double *g, double *x, int CC,
                int64_t N, int64_t X, int64_t Y, int64_t Z, const int *bound,
                const double *voxel_size, double absolute, double membrane, double bending,
-               double lame_shear, double lame_div, int max_iter, double tol)
+               double lame_shear, double lame_div, int max_iter, double tol, const double *w)
 {
   int64_t n = N * 3 * X * Y * Z;
   double *r = malloc(sizeof(double) * n), *p = malloc(sizeof(double) * n), *q = malloc(sizeof(double) * n);
-  or_regulariser_grid(x, q, h, CC, N, X, Y, Z, bound, voxel_size, absolute, membrane, bending, lame_shear, lame_div);
+  or_regulariser_grid(x, q, h, CC, N, X, Y, Z, bound, voxel_size, absolute, membrane, bending, lame_shear, lame_div, w);
   double rr = 0, gg = 0;
   for (int64_t i = 0; i < n; ++i) { r[i] = g[i] - q[i]; p[i] = r[i]; rr += r[i] * r[i]; gg += g[i] * g[i]; }
   int it = 0;
   for (; it < max_iter && sqrt(rr) > tol * sqrt(gg); ++it) {
-    or_regulariser_grid(p, q, h, CC, N, X, Y, Z, bound, voxel_size, absolute, membrane, bending, lame_shear, lame_div);
+    or_regulariser_grid(p, q, h, CC, N, X, Y, Z, bound, voxel_size, absolute, membrane, bending, lame_shear, lame_div, w);
     double pq = 0;
     for (int64_t i = 0; i < n; ++i) pq += p[i] * q[i];
     double a = rr / pq, rr2 = 0;
