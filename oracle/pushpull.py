@@ -8,6 +8,13 @@ Arrays follow the reference's layouts: input `[N, C, *spatial]`, grid
 `[N, *spatial_out, dim]`.  `bound` / `order` are lists padded to three
 entries by repeating the last one (pushpull_common.cpp:96-110); bound codes
 are the reference's BoundType values (bounds.h:7-17).
+
+`oob_f32=True` makes the FP64 restatement decide which samples are out of
+bounds the way the reference does for float32 data: in float arithmetic,
+`x >= -tol && x < (float)(n - 1) + tol` with `tol = (float)TINY`
+(bounds_common.h:168-171, pushpull_common.cpp:900).  For float32
+coordinates that sit exactly on an edge (e.g. float32(44.05) with n = 45)
+the FP64 test and the float test disagree; weights and sums stay FP64.
 """
 import ctypes
 import os
@@ -63,7 +70,7 @@ def _ptr(a):
 
 
 def _call(dtype, dim, bound, order, extrapolate, flags, N, C, src_size,
-          trgt_size, trgt_K, src, trgt, grid, out, gout, quirk_zorder):
+          trgt_size, trgt_K, src, trgt, grid, out, gout, quirk_zorder, oob_f32=False):
     sfx = "_f64" if dtype == np.float64 else "_f32"
     fn = getattr(lib(), "or_pushpull" + sfx)
     b = (ctypes.c_int * 3)(*bound_codes(bound))
@@ -74,7 +81,7 @@ def _call(dtype, dim, bound, order, extrapolate, flags, N, C, src_size,
             *[ctypes.c_int(int(f)) for f in flags],
             ctypes.c_int64(N), ctypes.c_int64(C), ss, ts, ctypes.c_int(trgt_K),
             _ptr(src), _ptr(trgt), _ptr(grid), _ptr(out), _ptr(gout),
-            ctypes.c_int(int(quirk_zorder)))
+            ctypes.c_int(int(bool(quirk_zorder)) | (2 if oob_f32 else 0)))
     if rc != 0:
         raise RuntimeError("oracle pushpull failed")
 
@@ -83,7 +90,7 @@ def _prep(*arrs, dtype):
     return [None if a is None else np.ascontiguousarray(a, dtype=dtype) for a in arrs]
 
 
-def grid_pull(input, grid, bound, order, extrapolate=0, quirk_zorder=False):
+def grid_pull(input, grid, bound, order, extrapolate=0, quirk_zorder=False, oob_f32=False):
     """pushpull.cpp:75-107 — returns [N, C, *grid_spatial]."""
     dtype = input.dtype
     input, grid = _prep(input, grid, dtype=dtype)
@@ -92,11 +99,11 @@ def grid_pull(input, grid, bound, order, extrapolate=0, quirk_zorder=False):
     out = np.empty((N, C) + grid.shape[1:-1], dtype)
     _call(dtype, dim, bound, order, extrapolate, (1, 0, 0, 0, 0), N, C,
           input.shape[2:], grid.shape[1:-1], 0, input, None, grid, out, None,
-          quirk_zorder)
+          quirk_zorder, oob_f32)
     return out
 
 
-def grid_grad(input, grid, bound, order, extrapolate=0, quirk_zorder=False):
+def grid_grad(input, grid, bound, order, extrapolate=0, quirk_zorder=False, oob_f32=False):
     """pushpull.cpp:268-300 — returns [N, C, *grid_spatial, dim]."""
     dtype = input.dtype
     input, grid = _prep(input, grid, dtype=dtype)
@@ -105,11 +112,11 @@ def grid_grad(input, grid, bound, order, extrapolate=0, quirk_zorder=False):
     out = np.empty((N, C) + grid.shape[1:-1] + (dim,), dtype)
     _call(dtype, dim, bound, order, extrapolate, (0, 0, 0, 0, 1), N, C,
           input.shape[2:], grid.shape[1:-1], 0, input, None, grid, out, None,
-          quirk_zorder)
+          quirk_zorder, oob_f32)
     return out
 
 
-def grid_push(input, grid, shape, bound, order, extrapolate=0, quirk_zorder=False):
+def grid_push(input, grid, shape, bound, order, extrapolate=0, quirk_zorder=False, oob_f32=False):
     """pushpull.cpp:131-184 — input [N, C, *grid_spatial] -> [N, C, *shape]."""
     dtype = input.dtype
     input, grid = _prep(input, grid, dtype=dtype)
@@ -119,11 +126,11 @@ def grid_push(input, grid, shape, bound, order, extrapolate=0, quirk_zorder=Fals
     out = np.empty((N, C) + shape, dtype)
     _call(dtype, dim, bound, order, extrapolate, (0, 1, 0, 0, 0), N, C,
           shape, grid.shape[1:-1], 0, None, input, grid, out, None,
-          quirk_zorder)
+          quirk_zorder, oob_f32)
     return out
 
 
-def grid_count(grid, shape, bound, order, extrapolate=0, quirk_zorder=False):
+def grid_count(grid, shape, bound, order, extrapolate=0, quirk_zorder=False, oob_f32=False):
     """pushpull.cpp:204-247 — returns [N, 1, *shape]."""
     dtype = grid.dtype
     (grid,) = _prep(grid, dtype=dtype)
@@ -133,12 +140,12 @@ def grid_count(grid, shape, bound, order, extrapolate=0, quirk_zorder=False):
     out = np.empty((N, 1) + shape, dtype)
     _call(dtype, dim, bound, order, extrapolate, (0, 0, 1, 0, 0), N, 1,
           shape, grid.shape[1:-1], 0, None, None, grid, out, None,
-          quirk_zorder)
+          quirk_zorder, oob_f32)
     return out
 
 
 def grid_pull_backward(grad, input, grid, bound, order, extrapolate=0,
-                       need_input=True, need_grid=True, quirk_zorder=False):
+                       need_input=True, need_grid=True, quirk_zorder=False, oob_f32=False):
     """pushpull.cpp:109-128 — (grad_input, grad_grid); None where not needed."""
     dtype = input.dtype
     grad, input, grid = _prep(grad, input, grid, dtype=dtype)
@@ -149,12 +156,12 @@ def grid_pull_backward(grad, input, grid, bound, order, extrapolate=0,
     _call(dtype, dim, bound, order, extrapolate,
           (0, int(need_input), 0, int(need_grid), 0), N, C,
           input.shape[2:], grid.shape[1:-1], 0, input, grad, grid, gi, gg,
-          quirk_zorder)
+          quirk_zorder, oob_f32)
     return gi, gg
 
 
 def grid_push_backward(grad, input, grid, bound, order, extrapolate=0,
-                       need_input=True, need_grid=True, quirk_zorder=False):
+                       need_input=True, need_grid=True, quirk_zorder=False, oob_f32=False):
     """pushpull.cpp:186-202 — source=grad (push output space), target=input."""
     dtype = input.dtype
     grad, input, grid = _prep(grad, input, grid, dtype=dtype)
@@ -165,11 +172,11 @@ def grid_push_backward(grad, input, grid, bound, order, extrapolate=0,
     _call(dtype, dim, bound, order, extrapolate,
           (int(need_input), 0, 0, int(need_grid), 0), N, C,
           grad.shape[2:], grid.shape[1:-1], 0, grad, input, grid, gi, gg,
-          quirk_zorder)
+          quirk_zorder, oob_f32)
     return gi, gg
 
 
-def grid_count_backward(grad, grid, bound, order, extrapolate=0, quirk_zorder=False):
+def grid_count_backward(grad, grid, bound, order, extrapolate=0, quirk_zorder=False, oob_f32=False):
     """pushpull.cpp:249-265 — grid gradient of count; source=grad, no target."""
     dtype = grid.dtype
     grad, grid = _prep(grad, grid, dtype=dtype)
@@ -178,12 +185,12 @@ def grid_count_backward(grad, grid, bound, order, extrapolate=0, quirk_zorder=Fa
     gg = np.empty(grid.shape, dtype)
     _call(dtype, dim, bound, order, extrapolate, (0, 0, 0, 1, 0), N,
           grad.shape[1], grad.shape[2:], grid.shape[1:-1], 0, grad, None,
-          grid, None, gg, quirk_zorder)
+          grid, None, gg, quirk_zorder, oob_f32)
     return gg
 
 
 def grid_grad_backward(grad, input, grid, bound, order, extrapolate=0,
-                       need_input=True, need_grid=True, quirk_zorder=False):
+                       need_input=True, need_grid=True, quirk_zorder=False, oob_f32=False):
     """pushpull.cpp:303-327 — target = grad [N, C, *out, dim] (trgt_K = dim)."""
     dtype = input.dtype
     grad, input, grid = _prep(grad, input, grid, dtype=dtype)
@@ -194,7 +201,7 @@ def grid_grad_backward(grad, input, grid, bound, order, extrapolate=0,
     _call(dtype, dim, bound, order, extrapolate,
           (0, int(need_input), 0, int(need_grid), 0), N, C,
           input.shape[2:], grid.shape[1:-1], dim, input, grad, grid, gi, gg,
-          quirk_zorder)
+          quirk_zorder, oob_f32)
     return gi, gg
 
 

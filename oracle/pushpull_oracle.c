@@ -118,6 +118,9 @@ static ldbl bspl(int n, ldbl x)
   if (n == 0) return fabsl(x) < 0.5L ? 1.0L : 0.0L;
   ldbl s = 0, fact = 1;
   for (int i = 2; i <= n; ++i) fact *= i;
+  /* last piece, (half - |x|)^n / n!: the truncated-power sum below cancels
+   * there, and would leave tiny weights with an absolute error only */
+  if (fabsl(x) >= half - 1) return powl(half - fabsl(x), n) / fact;
   for (int k = 0; k <= n + 1; ++k) {
     ldbl t = x + half - k;
     if (t > 0) s += ((k & 1) ? -1 : 1) * binom(n + 1, k) * powl(t, n);
@@ -203,15 +206,20 @@ static void unit_taps(taps_t *t)
  * zero-initialises push/count/grad outputs like the reference's allocator
  * (pushpull_common.cpp:475-571).
  *
- * quirk_zorder != 0 reproduces pushpull_common.cpp:591 (the generic path uses
- * the y-axis order for z when orders are not all equal).
+ * quirks, a bit set:
+ *   1  reproduces pushpull_common.cpp:591 (the generic path uses the y-axis
+ *      order for z when orders are not all equal);
+ *   2  decides out-of-bound samples in float arithmetic, as the reference
+ *      does for float32 data (inbounds<float>, bounds_common.h:168-171, with
+ *      tol = (float)TINY, pushpull_common.cpp:900); weights and sums keep
+ *      `real`.
  */
 int OR_FN(or_pushpull)(
     int dim, const int *bound, const int *order, int extrapolate,
     int do_pull, int do_push, int do_count, int do_grad, int do_sgrad,
     int64_t N, int64_t C, const int64_t *src_size, const int64_t *trgt_size,
     int trgt_K, const real *src, const real *trgt, const real *grid,
-    real *out, real *gout, int quirk_zorder)
+    real *out, real *gout, int quirks)
 {
   int64_t X = src_size[0], Y = dim > 1 ? src_size[1] : 1, Z = dim > 2 ? src_size[2] : 1;
   int64_t Xt = trgt_size[0], Yt = dim > 1 ? trgt_size[1] : 1, Zt = dim > 2 ? trgt_size[2] : 1;
@@ -221,7 +229,7 @@ int OR_FN(or_pushpull)(
   for (int d = 0; d < 3; ++d) { b[d] = bound[d]; o[d] = order[d]; }
   int iso = o[0] == o[1] && o[0] == o[2];
   int ogen[3] = {o[0], o[1], o[2]};
-  if (quirk_zorder && !iso && dim == 3) ogen[2] = o[1];
+  if ((quirks & 1) && !iso && dim == 3) ogen[2] = o[1];
   int64_t vin = X * Y * Z, vout = Xt * Yt * Zt;
   int64_t nout_c = do_sgrad ? (int64_t)dim : 1;
 
@@ -242,8 +250,14 @@ int OR_FN(or_pushpull)(
     int64_t sz[3] = {X, Y, Z};
     /* check{1,2,3}d: pushpull_common.cpp:925-1071 */
     int inb = 1;
-    for (int d = 0; d < dim; ++d)
-      inb = inb && (xs[d] >= -tol && xs[d] < (real)(sz[d] - 1) + tol);
+    for (int d = 0; d < dim; ++d) {
+      if (quirks & 2) {
+        const float xf = (float)xs[d], tf = (float)tol;
+        inb = inb && (xf >= -tf && xf < (float)(sz[d] - 1) + tf);
+      } else {
+        inb = inb && (xs[d] >= -tol && xs[d] < (real)(sz[d] - 1) + tol);
+      }
+    }
     if (!((extrapolate & 1) || inb)) {
       if (do_pull || do_sgrad)
         for (int64_t c = 0; c < C; ++c)

@@ -116,6 +116,20 @@ def test_spline_closed_forms():
     assert O.spline_weight(0, 0.49) == 1 and O.spline_weight(0, 0.5) == 0
 
 
+@pytest.mark.parametrize("order", range(1, 8))
+def test_spline_last_piece_relative_accuracy(order):
+    """On the last piece the B-spline is (h - |x|)^n / n! (h = (n + 1) / 2);
+    near the end of the support the weight is tiny and must keep its
+    relative accuracy (a truncated-power or Horner form cancels there)."""
+    import math
+    h = (order + 1) / 2
+    for eps in (1e-7, 1e-5, 1e-3, 0.1, 0.7):
+        for x in (h - eps, -(h - eps)):
+            s = h - abs(x)   # exact (Sterbenz)
+            want = s ** order / math.factorial(order)
+            assert abs(O.spline_weight(order, x) / want - 1) < 1e-13, (order, x)
+
+
 @pytest.mark.parametrize("bound,mode", [("dct2", "symmetric"), ("dct1", "reflect"),
                                         ("dft", "wrap"), ("replicate", "edge")])
 def test_bound_index_matches_numpy_pad(bound, mode):
@@ -328,6 +342,37 @@ def test_extrapolate2_tolerance_closed_form():
     p2 = O.grid_pull(x, grid, "replicate", 1, 2)[0, 0]
     assert p0[0] == 0 and p0[1] == 0 and p0[3] == 3.5
     assert p2[0] == 1.0 and p2[1] == 6.0 and p2[2] == 0 and p2[3] == 3.5
+
+
+GOLDEN_EDGE = os.path.join(HERE, "golden", "pushpull_golden_edge.npz")
+
+
+def test_golden_float_edges_oob_f32():
+    """float32 coordinates exactly on the out-of-bound edges (and their float
+    neighbours), extrapolate 0 and 2: the FP64 oracle with `oob_f32` (the
+    in-bound test in float, as the reference's float build does it) matches
+    the reference's float32 outputs, with the same zero pattern of the pull
+    (out-of-bound samples give exactly 0); the FP64 test alone disagrees on
+    some of these samples (tests/golden/make_golden_edge.py)."""
+    g = np.load(GOLDEN_EDGE)
+    n = n_diff = 0
+    for key, c in _cases(g):
+        dim, order, b, extrap = [int(v) for v in c["meta"]]
+        bound = BOUNDS[b]
+        inp, grid = c["inp"].astype(np.float64), c["grid"].astype(np.float64)
+        src = c["srcpush"].astype(np.float64)
+        shape = inp.shape[2:]
+        pull = O.grid_pull(inp, grid, bound, order, extrap, oob_f32=True)
+        assert np.array_equal(pull == 0, c["pull"] == 0), key
+        got = {"pull": pull,
+               "push": O.grid_push(src, grid, shape, bound, order, extrap, oob_f32=True),
+               "count": O.grid_count(grid, shape, bound, order, extrap, oob_f32=True)}
+        for name, out in got.items():
+            assert rel(out, c[name]) <= 2e-6, (key, name, order, bound, extrap)
+            n += 1
+        plain = O.grid_pull(inp, grid, bound, order, extrap)
+        n_diff += int(((plain == 0) != (pull == 0)).sum())
+    assert n == 216 and n_diff > 50, (n, n_diff)
 
 
 @pytest.mark.parametrize("order", range(8))
