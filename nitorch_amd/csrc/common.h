@@ -166,6 +166,51 @@ __host__ __device__ inline T first_tap_offset(int order) {
   return T(order - 1) * T(0.5);
 }
 
+// The outermost taps (k = 0 and k = O) lie on the spline's last piece,
+// (h - |d|)^O / O! with h = (O + 1) / 2.  Evaluated from the coordinate x
+// and the first tap lo: s = (lo + h) - x for k = 0, s = x - (lo + O - h)
+// for k = O.  Both differences are exact whenever s is small (Sterbenz), so
+// a tiny outer weight keeps the relative accuracy of T; computed as
+// h - |x - tap|, it would carry the rounding of x - tap (for |x| < 1 an ulp
+// of the tap distance, relative error 1e-3 on a weight of 1e-4 in float).
+__host__ __device__ constexpr double inv_fact(int n) {
+  return n <= 1 ? 1.0 : inv_fact(n - 1) / n;
+}
+
+template <typename T, typename I>
+__host__ __device__ inline T spl_outer_s(int O, int k, T x, I lo) {
+  return k == 0 ? (T(lo) + T(0.5 * (O + 1))) - x : x - (T(lo) + T(0.5 * (O - 1)));
+}
+
+// weight, first and second derivative (w.r.t. x) on the last piece
+template <typename T, int O>
+__host__ __device__ inline void spl_outer(bool left, T s, T &w, T &g, T &h) {
+  T p[O + 1];
+  p[0] = T(1);
+#pragma unroll
+  for (int i = 1; i <= O; ++i) p[i] = p[i - 1] * s;
+  // constexpr variables: the factorials fold at compile time (a call of
+  // the recursive inv_fact left in device code needs a dynamic stack)
+  constexpr double f0 = inv_fact(O), f1 = inv_fact(O - 1), f2 = inv_fact(O - 2);
+  w = p[O] * T(f0);
+  const T g0 = O >= 1 ? p[O >= 1 ? O - 1 : 0] * T(f1) : T(0);
+  g = left ? -g0 : g0;
+  h = O >= 2 ? p[O >= 2 ? O - 2 : 0] * T(f2) : T(0);
+}
+
+template <typename T>
+__host__ __device__ inline void spl_outer_rt(int order, bool left, T s, T &w, T &g, T &h) {
+  switch (order) {
+    case 1: spl_outer<T, 1>(left, s, w, g, h); break;
+    case 2: spl_outer<T, 2>(left, s, w, g, h); break;
+    case 3: spl_outer<T, 3>(left, s, w, g, h); break;
+    case 4: spl_outer<T, 4>(left, s, w, g, h); break;
+    case 5: spl_outer<T, 5>(left, s, w, g, h); break;
+    case 6: spl_outer<T, 6>(left, s, w, g, h); break;
+    default: spl_outer<T, 7>(left, s, w, g, h); break;
+  }
+}
+
 // --------------------------------------------------------------------------
 // Error reporting for the C ABI.
 // --------------------------------------------------------------------------

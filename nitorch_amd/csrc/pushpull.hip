@@ -143,9 +143,12 @@ static int pushpull_entry(int dim, const int32_t *bound, const int32_t *order, i
                           const int64_t *trgt_size, const void *src, const int64_t *src_stride,
                           const void *trgt, const int64_t *trgt_stride, const void *grid,
                           const int64_t *grid_stride, void *out, const int64_t *out_stride,
-                          void *gout, const int64_t *gout_stride, void *stream) {
+                          void *gout, const int64_t *gout_stride, void *stream,
+                          int *scratch = nullptr, int64_t scratch_cap = 0) {
   PPArgs<T> a;
   std::memset(&a, 0, sizeof(a));
+  a.scratch = scratch;
+  a.scratch_cap = scratch_cap;
   a.src = (const T *)src;
   a.trgt = (const T *)trgt;
   a.grid = (const T *)grid;

@@ -34,6 +34,11 @@ struct PPArgs {
   int dim;
   int extrap;
   int flags;
+  // caller-provided scratch for the tile path's list of scattered tiles
+  // (ints, capacity `scratch_cap`); null: the tile path allocates it on
+  // the stream
+  int *scratch;
+  int64_t scratch_cap;
 };
 
 #define NI_BLOCK 256
@@ -182,9 +187,9 @@ __device__ __forceinline__ void pp_sample(const PPArgs<T> &a, I n, I w, I h, I d
     nt[k] = ok + 1;
     if (lin) {
       // linear fast path weights (pushpull_common.cpp:1658-1667)
+      // (weights as distances to the far tap: exact when small)
       const I j0 = (I)dev_floor<T>(x);
-      const T d1 = x - T(j0);
-      tw[k][0] = T(1) - d1; tw[k][1] = d1;
+      tw[k][0] = T(j0 + 1) - x; tw[k][1] = x - T(j0);
       tg[k][0] = T(-1);     tg[k][1] = T(1);
       th[k][0] = T(0);      th[k][1] = T(0);
 #pragma unroll
@@ -202,7 +207,14 @@ __device__ __forceinline__ void pp_sample(const PPArgs<T> &a, I n, I w, I h, I d
         if (ORD < 0 && t > ok) break;
         const I b = lo + t;
         const T dd = x - T(b);
-        if constexpr (ORD >= 0) {
+        if (ok >= 1 && (t == 0 || t == ok)) {   // last piece, from x itself
+          const T s = spl_outer_s<T, I>(ok, t, x, lo);
+          T gw, hw;
+          if constexpr (ORD >= 1) spl_outer<T, (ORD >= 1 ? ORD : 1)>(t == 0, s, tw[k][t], gw, hw);
+          else spl_outer_rt<T>(ok, t == 0, s, tw[k][t], gw, hw);
+          tg[k][t] = gw;
+          th[k][t] = hw;
+        } else if constexpr (ORD >= 0) {
           tw[k][t] = spl_w<T, (ORD >= 0 ? ORD : 0)>(t, dd);
           if (need_g) tg[k][t] = spl_g<T, (ORD >= 0 ? ORD : 0)>(t, dd);
           if (need_h) th[k][t] = spl_h<T, (ORD >= 0 ? ORD : 0)>(t, dd);

@@ -8,20 +8,24 @@
 //  * gather modes (pull, gradients): stage the box of the source -- folded
 //            by the boundary condition and signed, so the taps need no
 //            folding -- in LDS, then read every tap from LDS;
-//  * push / count : accumulate the tile's splats into an LDS box with
-//            integer LDS atomics on a fixed-point scale chosen per tile and
-//            channel (2^e with  n_in * max|value| * 2^e < 2^30, so no sum can
-//            overflow; LDS float atomics are far slower than integer ones on
-//            gfx950), then add the box to the output once, folded and
+//  * push / count : accumulate the tile's splats into an LDS box of FP64
+//            values with FP64 LDS atomics (gfx950 runs ds_add_f64 at ~80% of
+//            the rate of its 64-bit integer atomics, and ~40x faster than
+//            ds_add_f32), then add the box to the output once, folded and
 //            signed, with one FP32 global atomic per non-zero box element.
-//            Rounding: <= 2^-(30 - log2 n_in) * max|value| per splat; the
-//            in-tile sum is exact and order-independent.
+//            Every box element is a double sum of its splats, so each output
+//            element keeps at least float's relative accuracy however small
+//            it is next to the tile's other values, and NaN / Inf values
+//            propagate as in the reference's float sums.
 // The box has fixed dimensions per order (every tap offset is then an
 // immediate of the LDS instruction); channels are staged NCC at a time.
 // A tile whose box does not fit (scattered or far out-of-range
-// coordinates with extrapolation) is left to a second kernel that runs the
-// direct per-sample routine pp_sample for its samples, so the result never
-// depends on the data's smoothness, only the speed does.
+// coordinates with extrapolation) is listed for a second kernel: tile_dyn
+// (pull / push / count) cuts it into sub-tiles with boxes sized to their own
+// spread, and runs the direct per-sample routine pp_sample for a sub-tile
+// that still does not fit; tile_fallback runs pp_sample for the gradient
+// modes.  The result never depends on the data's smoothness, only the speed
+// does.
 //
 // Weights, the out-of-bound rule and the splat order of each sample are
 // those of pp_sample (nitorch/_C/src/impl/pushpull_common.cpp: OOB :926-974,
@@ -78,13 +82,14 @@ struct TileBox {
   static constexpr int DYZ = DY * DZ;
   static constexpr int DYZP = DYZ + ((48 - DYZ) % 64 + 64) % 64;
   static constexpr int VOL = DX * DYZP;
+  static constexpr int VOL4 = DX * DYZ;   // pull: 4 interleaved channels per voxel, unpadded
   static constexpr int NPP = (DYZ + NTH - 1) / NTH;     // y-z plane positions per thread
-  // channels staged at once: 2 for the scatters (a 52 KB box at order 3)
-  // and for pull / grid gradient, 4 for grid_grad (its 3 accumulators per
+  // channels staged at once: 1 for the scatters (a 52 KB box of doubles
+  // at order 3) and 2 for pull / grid gradient, 4 for grid_grad (its 3 accumulators per
   // channel amortise the taps' weights and derivatives over more channels:
   // 4x8x192^3, order 3, 3.13 -> 2.67 ms, while pull itself is 1.57 ms at 2
   // and 1.75 ms at 4 channels)
-  static constexpr int MAX_NCC = SCAT ? 2 : 4;
+  static constexpr int MAX_NCC = SCAT ? 1 : 4;
   // waves per SIMD asked of the register allocator for the 512-thread
   // scatter: 6 (<= 85 VGPRs) lets three 52 KB boxes share a CU's LDS
   static constexpr int MINW = BIG ? 6 : 1;
@@ -166,14 +171,19 @@ __device__ __forceinline__ float tl_horner_g(float u) {
 template <int ORD, bool DERIV, int T>
 __device__ __forceinline__ void tl_weight(float x, int lo, float (&w)[ORD + 1], float (&g)[ORD + 1]) {
   if constexpr (T <= ORD) {
-    constexpr int P = tap_piece<ORD>(T);
-    constexpr float centre = (float)piece_centre<ORD>(P);
-    const float d = x - (float)(lo + T);
-    const float a = d < 0.f ? -d : d;
-    w[T] = tl_horner_w<ORD, P, 0>(a - centre);
-    if constexpr (DERIV) {
-      const float gg = tl_horner_g<ORD, P, 0>(a - centre);
-      g[T] = d < 0.f ? -gg : gg;
+    if constexpr (T == 0 || T == ORD) {   // last piece, from x itself (spl_outer_s)
+      float h;
+      spl_outer<float, ORD>(T == 0, spl_outer_s<float, int>(ORD, T, x, lo), w[T], g[T], h);
+    } else {
+      constexpr int P = tap_piece<ORD>(T);
+      constexpr float centre = (float)piece_centre<ORD>(P);
+      const float d = x - (float)(lo + T);
+      const float a = d < 0.f ? -d : d;
+      w[T] = tl_horner_w<ORD, P, 0>(a - centre);
+      if constexpr (DERIV) {
+        const float gg = tl_horner_g<ORD, P, 0>(a - centre);
+        g[T] = d < 0.f ? -gg : gg;
+      }
     }
     tl_weight<ORD, DERIV, T + 1>(x, lo, w, g);
   }
@@ -188,10 +198,10 @@ __device__ __forceinline__ int tl_taps(float x, float (&w)[ORD + 1], float (&g)[
     g[0] = 0.f;
     return (int)roundf(x);
   } else if constexpr (ORD == 1) {
+    // weights as distances to the far tap: exact when small
     const int j0 = (int)floorf(x);
-    const float d1 = x - (float)j0;
-    w[0] = 1.f - d1;
-    w[1] = d1;
+    w[0] = (float)(j0 + 1) - x;
+    w[1] = x - (float)j0;
     g[0] = -1.f;
     g[1] = 1.f;
     return j0;
@@ -250,10 +260,18 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
   constexpr bool GATHER = TileMeta<MODE>::gather, DERIV = TileMeta<MODE>::deriv;
   using B = TileBox<ORD, !GATHER>;
   constexpr int R = B::R;
-  constexpr int DY = B::DY, DZ = B::DZ, DYZ = B::DYZP, VOL = B::VOL;
+  // pull stages its box with the NCC = 4 channels of a voxel interleaved
+  // (one 16-byte LDS read per tap: gfx950's LDS serves 128-bit reads at
+  // ~1.5x the bytes per clock of 32-bit ones, and 16 z-consecutive lanes
+  // read 16 consecutive 16-byte voxels, which no bank serves twice);
+  // the other modes keep channel planes [c][x][y][z]
+  constexpr bool IL = MODE == TM_PULL;
+  constexpr int DY = B::DY, DZ = B::DZ, DYZ = IL ? B::DYZ : B::DYZP, VOL = IL ? B::VOL4 : B::VOL;
   static_assert(NCC >= 1 && NCC <= B::MAX_NCC, "channel chunk exceeds the LDS budget");
+  static_assert(!IL || NCC == 4 || NCC == 1, "pull interleaves 4 channels (or runs one)");
   constexpr int NTH = B::NTH;
-  __shared__ float box[NCC * VOL];
+  // gathers: FP32 source box; scatters: FP64 box (two words per element)
+  __shared__ __align__(16) float box[(GATHER ? 1 : 2) * NCC * VOL];
   __shared__ int red[B::NW][6];
 
   const int64_t tile = xcd_block(nblocks);
@@ -365,7 +383,7 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
     fits = fits && e <= (d == 0 ? B::DX : d == 1 ? DY : DZ);
     ext[d] = (int)(e < 0x7fffffff ? e : 0x7fffffff);
   }
-  if (!fits) {   // scattered tile: listed for tile_fallback
+  if (!fits) {   // scattered tile: listed for the second kernel
     if (fb && threadIdx.x == 0) fb[1 + atomicAdd(fb, 1)] = (int)tile;
     return;
   }
@@ -375,14 +393,6 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
     off[i] = inb[i] ? (lo[i][0] - mn[0]) * DYZ + (lo[i][1] - mn[1]) * DZ + (lo[i][2] - mn[2]) : 0;
 
   const int C = MODE == TM_COUNT ? 1 : (int)a.C;
-  // scatter: number of in-bound samples (bounds every fixed-point box sum)
-  int n_in = 0;
-  if (!GATHER) {
-#pragma unroll
-    for (int i = 0; i < R; ++i) n_in += __syncthreads_count(inb[i]);
-  }
-  __shared__ float s_red[B::NW][NCC];
-  __shared__ float s_scale[NCC], s_inv[NCC];
 
   // folded offsets (in elements of the staged / scattered tensor) and signs
   // of the box positions along each axis: fold() runs once per position
@@ -409,8 +419,8 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
   TilePlane<B> pl;
   pl.init(ext[1], ext[2], fofs, fsg);
   if (!GATHER) {   // scatter box: cleared once; each flush clears what it read
-    int *ibox = (int *)box;
-    for (int i = threadIdx.x; i < NCC * VOL; i += NTH) ibox[i] = 0;
+    double *dbox = (double *)box;
+    for (int i = threadIdx.x; i < NCC * VOL; i += NTH) dbox[i] = 0.0;
     __syncthreads();
   }
 
@@ -441,6 +451,28 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
       const float *src = a.src + (int)(n * a.src_s[0] + c0 * a.src_s[1]);
       const int sC = (int)a.src_s[1];
       const int e0 = ext[0] - 1;
+      if constexpr (IL && NCC == 4) {
+        // one x row of the 4 channels at a time: 4 DX loads, then DX
+        // 16-byte stores (channels >= nc of a partial chunk stage zeros)
+#pragma unroll
+        for (int j = 0; j < B::NPP; ++j) {
+          if (pl.yz_lds[j] < 0) continue;
+          const float *sp = src + pl.yz_ofs[j];
+          float v[4][B::DX];
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int bx = 0; bx < B::DX; ++bx) v[c][bx] = c < nc ? sp[c * sC + fofs[bx < e0 ? bx : e0]] : 0.f;
+          float4 *bq = (float4 *)box + pl.yz_lds[j];
+#pragma unroll
+          for (int bx = 0; bx < B::DX; ++bx) {
+            const float sg = fsg[bx] * pl.yz_sg[j];
+            if (bx <= e0)
+              bq[bx * DYZ] = sg != 0.f ? make_float4(v[0][bx] * sg, v[1][bx] * sg, v[2][bx] * sg, v[3][bx] * sg)
+                                       : make_float4(0.f, 0.f, 0.f, 0.f);
+          }
+        }
+      } else
 #pragma unroll
       for (int j = 0; j < B::NPP; ++j) {
         if (pl.yz_lds[j] < 0) continue;
@@ -459,12 +491,9 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
         }
       }
     }
-    // scatter: values of this chunk and the per-channel fixed-point scales
+    // scatter: values of this chunk
     float tv[R][NCC];
     if (!GATHER) {
-      float mloc[NCC];
-#pragma unroll
-      for (int c = 0; c < NCC; ++c) mloc[c] = 0.f;
       const float *tp0 = MODE == TM_PUSH ? a.trgt + (int)(n * a.trgt_s[0] + (int64_t)c0 * a.trgt_s[1] +
                                                           px * a.trgt_s[2] + py * a.trgt_s[3] + pz0 * a.trgt_s[4])
                                          : nullptr;
@@ -480,30 +509,6 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
         } else {
           tv[i][0] = 1.f;
         }
-#pragma unroll
-        for (int c = 0; c < NCC; ++c) mloc[c] = fmaxf(mloc[c], fabsf(tv[i][c]));
-      }
-#pragma unroll
-      for (int c = 0; c < NCC; ++c) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) mloc[c] = fmaxf(mloc[c], __shfl_xor(mloc[c], o, 64));
-        if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6][c] = mloc[c];
-      }
-      __syncthreads();
-      if (threadIdx.x < NCC) {
-        float m = s_red[0][threadIdx.x];
-#pragma unroll
-        for (int wv = 1; wv < B::NW; ++wv) m = fmaxf(m, s_red[wv][threadIdx.x]);
-        int e = 0;
-        if (m > 0.f && n_in > 0) {
-          int q, kb = 0;
-          frexpf(m, &q);                       // m < 2^q
-          while ((1 << kb) < n_in) ++kb;       // n_in <= 2^kb
-          e = 30 - q - kb;
-          e = e > 126 ? 126 : (e < -126 ? -126 : e);
-        }
-        s_scale[threadIdx.x] = ldexpf(1.f, e);
-        s_inv[threadIdx.x] = ldexpf(1.f, -e);
       }
     }
     __syncthreads();
@@ -518,22 +523,44 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
 #pragma unroll
         for (int i = 0; i < R; ++i) {
           if (!inb[i]) continue;
-          const float *bp = box + off[i];
+          if constexpr (NCC == 4) {
+            const float4 *bp = (const float4 *)box + off[i];
 #pragma unroll 1
-          for (int kx = 0; kx < NT; ++kx, bp += DYZ) {
+            for (int kx = 0; kx < NT; ++kx, bp += DYZ) {
 #pragma unroll
-            for (int ky = 0; ky < NT; ++ky) {
-              const float wxy = w[i][0][0] * w[i][1][ky];
+              for (int ky = 0; ky < NT; ++ky) {
+                const float wxy = w[i][0][0] * w[i][1][ky];
 #pragma unroll
-              for (int kz = 0; kz < NT; ++kz) {
-                const float ww = wxy * w[i][2][kz];
-                const int o = ky * DZ + kz;
-#pragma unroll
-                for (int c = 0; c < NCC; ++c) acc[i][c] += bp[c * VOL + o] * ww;
+                for (int kz = 0; kz < NT; ++kz) {
+                  const float ww = wxy * w[i][2][kz];
+                  const float4 v = bp[ky * DZ + kz];
+                  acc[i][0] += v.x * ww;
+                  acc[i][1] += v.y * ww;
+                  acc[i][2] += v.z * ww;
+                  acc[i][3] += v.w * ww;
+                }
+                __builtin_amdgcn_sched_barrier(0);
               }
-              __builtin_amdgcn_sched_barrier(0);
+              tl_rot(w[i][0]);
             }
-            tl_rot(w[i][0]);
+          } else {
+            const float *bp = box + off[i];
+#pragma unroll 1
+            for (int kx = 0; kx < NT; ++kx, bp += DYZ) {
+#pragma unroll
+              for (int ky = 0; ky < NT; ++ky) {
+                const float wxy = w[i][0][0] * w[i][1][ky];
+#pragma unroll
+                for (int kz = 0; kz < NT; ++kz) {
+                  const float ww = wxy * w[i][2][kz];
+                  const int o = ky * DZ + kz;
+#pragma unroll
+                  for (int c = 0; c < NCC; ++c) acc[i][c] += bp[c * VOL + o] * ww;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+              }
+              tl_rot(w[i][0]);
+            }
           }
         }
       }
@@ -685,18 +712,17 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
         og[R - 1][2] = o2;
       }
     } else {
-      // push / count: fixed-point splats (exact: the scale is a power of
-      // two), one sample at a time (rolled): element 0 of the per-sample
-      // state is the current sample, rotated after each (R rotations
-      // restore it for the next channel chunk)
+      // push / count: FP64 splats, one sample at a time (rolled): element 0
+      // of the per-sample state is the current sample, rotated after each
+      // (R rotations restore it for the next channel chunk)
 #pragma unroll 1
       for (int i = 0; i < R; ++i) {
         const bool ii = inb[0];
         const int oi = off[0];
         const float q0 = xq[0][0], q1 = xq[0][1], q2 = xq[0][2];
-        float svi[NCC];
+        double vd[NCC];
 #pragma unroll
-        for (int c = 0; c < NCC; ++c) svi[c] = tv[0][c] * s_scale[c];
+        for (int c = 0; c < NCC; ++c) vd[c] = (double)tv[0][c];
         tl_rot(inb);
         tl_rot(off);
 #pragma unroll
@@ -718,18 +744,26 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
         tl_taps<ORD>(q0, wi[0], gi[0]);
         tl_taps<ORD>(q1, wi[1], gi[1]);
         tl_taps<ORD>(q2, wi[2], gi[2]);
-        int *bp = (int *)box + oi;
+        double wy[NT], wz[NT];
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+          wy[k] = (double)wi[1][k];
+          wz[k] = (double)wi[2][k];
+        }
+        double *bp = (double *)box + oi;
 #pragma unroll 1
         for (int kx = 0; kx < NT; ++kx, bp += DYZ) {
 #pragma unroll
           for (int ky = 0; ky < NT; ++ky) {
-            const float wxy = wi[0][0] * wi[1][ky];
+            const double wxy = (double)wi[0][0] * wy[ky];
+            double t[NCC];
+#pragma unroll
+            for (int c = 0; c < NCC; ++c) t[c] = wxy * vd[c];
 #pragma unroll
             for (int kz = 0; kz < NT; ++kz) {
-              const float ww = wxy * wi[2][kz];
               const int o = ky * DZ + kz;
 #pragma unroll
-              for (int c = 0; c < NCC; ++c) atomicAdd(bp + c * VOL + o, (int)rintf(ww * svi[c]));
+              for (int c = 0; c < NCC; ++c) atomicAdd(bp + c * VOL + o, t[c] * wz[kz]);
             }
           }
           tl_rot(wi[0]);
@@ -740,7 +774,7 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
       // the next chunk (the splats never leave the ext corner) ----
       __syncthreads();
       float *out = a.out + (int)(n * a.out_s[0] + (int64_t)c0 * oC);
-      int *ibox = (int *)box;
+      double *dbox = (double *)box;
       const int e0 = ext[0] - 1;
 #pragma unroll
       for (int j = 0; j < B::NPP; ++j) {
@@ -749,18 +783,17 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
         for (int c = 0; c < nc; ++c) {
           // a whole x row of the box read (and cleared) at once, then its
           // atomics issued back to back
-          int *bq = ibox + c * VOL + pl.yz_lds[j];
-          int iv[B::DX];
+          double *bq = dbox + c * VOL + pl.yz_lds[j];
+          float fv[B::DX];   // the row, rounded to float (0 where the box is empty)
 #pragma unroll
-          for (int bx = 0; bx < B::DX; ++bx) iv[bx] = bq[bx * DYZ];
+          for (int bx = 0; bx < B::DX; ++bx) fv[bx] = (float)bq[bx * DYZ];
 #pragma unroll
-          for (int bx = 0; bx < B::DX; ++bx) bq[bx * DYZ] = 0;
+          for (int bx = 0; bx < B::DX; ++bx) bq[bx * DYZ] = 0.0;
           float *op = out + c * oC + pl.yz_ofs[j];
-          const float inv = s_inv[c];
 #pragma unroll
           for (int bx = 0; bx < B::DX; ++bx) {
             const float sg = fsg[bx] * pl.yz_sg[j];
-            if (bx <= e0 && iv[bx] != 0 && sg != 0.f) atomic_add(op + fofs[bx], ((float)iv[bx] * inv) * sg);
+            if (bx <= e0 && fv[bx] != 0.f && sg != 0.f) atomic_add(op + fofs[bx], fv[bx] * sg);
             if ((bx & 3) == 3) __builtin_amdgcn_sched_barrier(0);   // bound the addresses held at once
           }
         }
@@ -820,7 +853,7 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH)) void 
   const int n = (int)(r / ntx);
   const int sz[3] = {(int)a.sX, (int)a.sY, (int)a.sZ};
   const int px = tx * B::TX + sx, py = ty * B::TY + sy, pz0 = tz * B::TZ + sz0;
-  bool valid[R];
+  bool valid[R], inb[R];
   int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
   const float tol = (a.extrap & 2) ? 0.55f : 0.05f;
   const int gD = (int)a.grid_s[4];
@@ -829,6 +862,7 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH)) void 
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     valid[i] = px < X && py + i * B::SDY < Y && pz0 + i * B::SDZ < Z;
+    inb[i] = false;
     if (valid[i]) {
       const float *gp = gp0 + i * gS;
       const float xs[3] = {gp[0], gp[gD], gp[2 * gD]};
@@ -836,6 +870,7 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH)) void 
       if (!(a.extrap & 1))
 #pragma unroll
         for (int d = 0; d < 3; ++d) ok = ok && (xs[d] >= -tol && xs[d] < (float)(sz[d] - 1) + tol);
+      inb[i] = ok;
       if (ok) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -879,6 +914,233 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH)) void 
     if (valid[i]) pp_sample<float, int32_t, ORD, FL, true>(a, n, px, py + i * B::SDY, pz0 + i * B::SDZ);
 }
 
+// Tiles that tile_kernel listed (taps too spread for the static box): each
+// listed tile is cut into sub-tiles of
+// 4 x 8 x 8 samples (one per thread of a 256-thread work-group), and each
+// sub-tile stages / accumulates an LDS box sized from its own first-tap
+// spread (runtime strides, up to DYN_MAXE elements per channel).  Strongly
+// deformed grids (large rotations, zooms, displacements with a large
+// Jacobian) keep most of the tile path's reuse this way: a sub-tile's box is
+// a few times its sample count instead of the per-sample kernel's 64 taps
+// x channels of scattered global traffic.  A sub-tile whose box is still
+// too large runs the per-sample routine.  Same weights, out-of-bound rule,
+// box fill and FP64 accumulation as tile_kernel (pull / push / count);
+// other modes take the per-sample kernel for listed tiles.
+#define DYN_TX 4
+#define DYN_TY 8
+#define DYN_TZ 8
+#define DYN_NTH 256
+#define DYN_MAXE 8192        // box elements per channel (32 KB of floats, 64 KB of doubles)
+#define DYN_MAXAX 128        // box extent per axis
+
+template <int ORD, int MODE>
+__global__ __launch_bounds__(DYN_NTH) void tile_dyn(PPArgs<float> a, int ntx, int nty, int ntz, const int *fb) {
+  constexpr int NT = ORD + 1;
+  constexpr bool GATHER = MODE == TM_PULL;
+  constexpr int FL = TileFlags<MODE>::value;
+  using B = TileBox<ORD, !GATHER>;
+  constexpr int NCC = GATHER ? 2 : 1;
+  constexpr int NW = DYN_NTH / 64;
+  static_assert(DYN_TX * DYN_TY * DYN_TZ == DYN_NTH, "one sample per thread");
+  static_assert(B::TX % DYN_TX == 0 && B::TY % DYN_TY == 0 && B::TZ % DYN_TZ == 0, "sub-tiles tile the tile");
+  constexpr int NSX = B::TX / DYN_TX, NSY = B::TY / DYN_TY, NSZ = B::TZ / DYN_TZ;
+  // gathers: NCC FP32 channels; scatters: one channel of doubles
+  __shared__ __align__(16) float box[2 * DYN_MAXE];
+  __shared__ int fofs[3 * DYN_MAXAX];
+  __shared__ float fsg[3 * DYN_MAXAX];
+  __shared__ int red[NW][6];
+
+  const int X = (int)a.tX, Y = (int)a.tY, Z = (int)a.tZ;
+  const int sz[3] = {(int)a.sX, (int)a.sY, (int)a.sZ};
+  const float tol = (a.extrap & 2) ? 0.55f : 0.05f;
+  const int lx = threadIdx.x / (DYN_TY * DYN_TZ), ly = (threadIdx.x / DYN_TZ) % DYN_TY, lz = threadIdx.x % DYN_TZ;
+  const int cnt = fb[0];
+  for (int64_t j = blockIdx.x; j < (int64_t)cnt * NSX * NSY * NSZ; j += gridDim.x) {
+    const int sub = (int)(j % (NSX * NSY * NSZ));
+    int64_t r = fb[1 + j / (NSX * NSY * NSZ)];
+    const int tz = (int)(r % ntz); r /= ntz;
+    const int ty = (int)(r % nty); r /= nty;
+    const int tx = (int)(r % ntx);
+    const int n = (int)(r / ntx);
+    const int px = tx * B::TX + (sub / (NSY * NSZ)) * DYN_TX + lx;
+    const int py = ty * B::TY + ((sub / NSZ) % NSY) * DYN_TY + ly;
+    const int pz = tz * B::TZ + (sub % NSZ) * DYN_TZ + lz;
+    const bool valid = px < X && py < Y && pz < Z;
+    bool inb = false;
+    float xq[3] = {0.f, 0.f, 0.f};
+    int lo[3] = {0, 0, 0};
+    int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
+    if (valid) {
+      const float *gp = a.grid + (int64_t)n * a.grid_s[0] + (int64_t)px * a.grid_s[1] + (int64_t)py * a.grid_s[2] +
+                        (int64_t)pz * a.grid_s[3];
+      const int gD = (int)a.grid_s[4];
+      xq[0] = gp[0]; xq[1] = gp[gD]; xq[2] = gp[2 * gD];
+      bool ok = true;
+      if (!(a.extrap & 1))
+#pragma unroll
+        for (int d = 0; d < 3; ++d) ok = ok && (xq[d] >= -tol && xq[d] < (float)(sz[d] - 1) + tol);
+      inb = ok;
+      if (ok) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          float wt[NT], gt[NT];
+          lo[d] = tl_taps<ORD>(xq[d], wt, gt);
+          mn[d] = lo[d];
+          mx[d] = lo[d];
+        }
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      mn[d] = wave_min(mn[d]);
+      mx[d] = wave_max(mx[d]);
+    }
+    __syncthreads();   // previous sub-tile done with red / box / tables
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        red[threadIdx.x >> 6][d] = mn[d];
+        red[threadIdx.x >> 6][3 + d] = mx[d];
+      }
+    }
+    __syncthreads();
+    int ext[3];
+    bool none = false, fits = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      int m0 = red[0][d], m1 = red[0][3 + d];
+#pragma unroll
+      for (int wv = 1; wv < NW; ++wv) {
+        m0 = min(m0, red[wv][d]);
+        m1 = max(m1, red[wv][3 + d]);
+      }
+      mn[d] = m0;
+      none = none || m0 > m1;
+      const int64_t e = (int64_t)m1 - (int64_t)m0 + NT;
+      fits = fits && e <= DYN_MAXAX;
+      ext[d] = fits && !none ? (int)e : 1;
+    }
+    if (none) {   // no in-bound sample: pull writes zeros
+      if (MODE == TM_PULL && valid) {
+        float *op = a.out + (int64_t)n * a.out_s[0] + (int64_t)px * a.out_s[2] + (int64_t)py * a.out_s[3] +
+                    (int64_t)pz * a.out_s[4];
+        for (int c = 0; c < (int)a.C; ++c) op[(int64_t)c * a.out_s[1]] = 0.f;
+      }
+      continue;
+    }
+    const int DZ = ext[2], DYZ = ext[1] * ext[2];
+    fits = fits && (int64_t)ext[0] * DYZ <= DYN_MAXE;
+    if (!fits) {   // per-sample routine for this sub-tile
+      if (valid) pp_sample<float, int32_t, ORD, FL, true>(a, n, px, py, pz);
+      continue;
+    }
+    const int VOL = ext[0] * DYZ;
+    // folded offsets and signs of the box positions along each axis
+    {
+      const int64_t *st = GATHER ? a.src_s : a.out_s;
+      for (int i = threadIdx.x; i < ext[0] + ext[1] + ext[2]; i += DYN_NTH) {
+        const int d = i < ext[0] ? 0 : (i < ext[0] + ext[1] ? 1 : 2);
+        const int b = i - (d == 0 ? 0 : (d == 1 ? ext[0] : ext[0] + ext[1]));
+        int sg = 0;
+        const int jj = fold<int>(a.bound[d], mn[d] + b, sz[d], sg);
+        fofs[i] = jj * (int)st[2 + d];
+        fsg[i] = (float)sg;
+      }
+    }
+    // this sample's box offset and weights
+    float w[3][NT], g[3][NT];
+    int off = 0;
+    if (inb) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) tl_taps<ORD>(xq[d], w[d], g[d]);
+      off = (lo[0] - mn[0]) * DYZ + (lo[1] - mn[1]) * DZ + (lo[2] - mn[2]);
+    }
+    __syncthreads();
+    if (GATHER) {
+      const float *src = a.src + (int64_t)n * a.src_s[0];
+      float *op = a.out + (int64_t)n * a.out_s[0] + (int64_t)px * a.out_s[2] + (int64_t)py * a.out_s[3] +
+                  (int64_t)pz * a.out_s[4];
+      for (int c0 = 0; c0 < (int)a.C; c0 += NCC) {
+        const int nc = min(NCC, (int)a.C - c0);
+        if (c0) __syncthreads();
+        // stage the folded, signed box of channels c0 .. c0 + nc
+        for (int q = threadIdx.x; q < VOL; q += DYN_NTH) {
+          const int bx = q / DYZ, rr = q - bx * DYZ, by = rr / DZ, bz = rr - by * DZ;
+          const float sg = fsg[bx] * fsg[ext[0] + by] * fsg[ext[0] + ext[1] + bz];
+          const int so = fofs[bx] + fofs[ext[0] + by] + fofs[ext[0] + ext[1] + bz];
+#pragma unroll
+          for (int c = 0; c < NCC; ++c)
+            if (c < nc) box[c * DYN_MAXE + q] = sg != 0.f ? src[(int64_t)(c0 + c) * a.src_s[1] + so] * sg : 0.f;
+        }
+        __syncthreads();
+        if (valid) {
+          float acc[NCC];
+#pragma unroll
+          for (int c = 0; c < NCC; ++c) acc[c] = 0.f;
+          if (inb) {
+            const float *bp = box + off;
+#pragma unroll 1
+            for (int kx = 0; kx < NT; ++kx, bp += DYZ) {
+#pragma unroll
+              for (int ky = 0; ky < NT; ++ky) {
+                const float wxy = w[0][kx] * w[1][ky];
+                const float *bq = bp + ky * DZ;
+#pragma unroll
+                for (int kz = 0; kz < NT; ++kz) {
+                  const float ww = wxy * w[2][kz];
+#pragma unroll
+                  for (int c = 0; c < NCC; ++c) acc[c] += bq[c * DYN_MAXE + kz] * ww;
+                }
+              }
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < NCC; ++c)
+            if (c < nc) op[(int64_t)(c0 + c) * a.out_s[1]] = acc[c];
+        }
+      }
+    } else {
+      double *dbox = (double *)box;
+      for (int q = threadIdx.x; q < VOL; q += DYN_NTH) dbox[q] = 0.0;
+      const float *tq = MODE == TM_PUSH ? a.trgt + (int64_t)n * a.trgt_s[0] + (int64_t)px * a.trgt_s[2] +
+                                              (int64_t)py * a.trgt_s[3] + (int64_t)pz * a.trgt_s[4]
+                                        : nullptr;
+      float *out = a.out + (int64_t)n * a.out_s[0];
+      const int C = MODE == TM_COUNT ? 1 : (int)a.C;
+      for (int c = 0; c < C; ++c) {
+        const float v = !inb ? 0.f : (MODE == TM_PUSH ? tq[(int64_t)c * a.trgt_s[1]] : 1.f);
+        __syncthreads();   // box cleared (first channel) / flushed (later ones)
+        if (inb && v != 0.f) {
+          const double vd = (double)v;
+          double *bp = dbox + off;
+#pragma unroll 1
+          for (int kx = 0; kx < NT; ++kx, bp += DYZ) {
+#pragma unroll
+            for (int ky = 0; ky < NT; ++ky) {
+              const double t = (double)w[0][kx] * (double)w[1][ky] * vd;
+              double *bq = bp + ky * DZ;
+#pragma unroll
+              for (int kz = 0; kz < NT; ++kz) atomicAdd(bq + kz, t * (double)w[2][kz]);
+            }
+          }
+        }
+        __syncthreads();
+        // flush: one FP32 global atomic per non-zero box element; clear it
+        float *oc = out + (int64_t)c * a.out_s[1];
+        for (int q = threadIdx.x; q < VOL; q += DYN_NTH) {
+          const double dv = dbox[q];
+          dbox[q] = 0.0;
+          if (dv == 0.0) continue;
+          const int bx = q / DYZ, rr = q - bx * DYZ, by = rr / DZ, bz = rr - by * DZ;
+          const float sg = fsg[bx] * fsg[ext[0] + by] * fsg[ext[0] + ext[1] + bz];
+          if (sg != 0.f)
+            atomic_add(oc + fofs[bx] + fofs[ext[0] + by] + fofs[ext[0] + ext[1] + bz], (float)dv * sg);
+        }
+      }
+    }
+  }
+}
+
 // tile_kernel, then tile_fallback on the tiles it listed as too scattered
 // (list in stream-ordered scratch; without it, tile_fallback repeats the
 // bounding-box test on every tile).
@@ -891,7 +1153,11 @@ static void launch_tile(const PPArgs<float> &a, hipStream_t st) {
   const int64_t nb = a.N * (int64_t)ntx * nty * ntz;
   const dim3 g((unsigned)nb), b(B::NTH);
   int *fb = nullptr;
-  if (hipMallocAsync((void **)&fb, sizeof(int) * (size_t)(nb + 1), st) != hipSuccess) {
+  const bool own = a.scratch == nullptr;
+  if (!own) {
+    fb = a.scratch;   // the caller checked nb + 1 <= a.scratch_cap
+    hipLaunchKernelGGL(fill_zero<int>, dim3(1), dim3(NI_BLOCK), 0, st, fb, (int64_t)1);
+  } else if (hipMallocAsync((void **)&fb, sizeof(int) * (size_t)(nb + 1), st) != hipSuccess) {
     (void)hipGetLastError();
     fb = nullptr;
   } else {
@@ -903,13 +1169,21 @@ static void launch_tile(const PPArgs<float> &a, hipStream_t st) {
     if (a.C == 1)
       hipLaunchKernelGGL((tile_kernel<ORD, MODE, 1>), g, b, 0, st, a, ntx, nty, ntz, nb, fb);
     else
-      hipLaunchKernelGGL((tile_kernel<ORD, MODE, MODE == TM_SGRAD ? B::MAX_NCC : 2>), g, b, 0, st, a, ntx, nty, ntz,
-                         nb, fb);
+      hipLaunchKernelGGL((tile_kernel<ORD, MODE, MODE == TM_SGRAD || MODE == TM_PUSH || MODE == TM_PULL ? B::MAX_NCC : 2>),
+                         g, b, 0, st, a, ntx, nty, ntz, nb, fb);
   }
   if (fb) {
-    const unsigned gf = (unsigned)(nb < 2048 ? nb : 2048);
-    hipLaunchKernelGGL((tile_fallback<ORD, MODE>), dim3(gf), b, 0, st, a, ntx, nty, ntz, nb, (const int *)fb);
-    (void)hipFreeAsync(fb, st);
+    if constexpr (MODE == TM_PULL || MODE == TM_PUSH || MODE == TM_COUNT) {
+      // listed tiles: sub-tiles with boxes sized to their spread
+      constexpr int nsub = (B::TX / DYN_TX) * (B::TY / DYN_TY) * (B::TZ / DYN_TZ);
+      const int64_t want = nb * nsub;
+      const unsigned gd = (unsigned)(want < 4096 ? want : 4096);
+      hipLaunchKernelGGL((tile_dyn<ORD, MODE>), dim3(gd), dim3(DYN_NTH), 0, st, a, ntx, nty, ntz, (const int *)fb);
+    } else {
+      const unsigned gf = (unsigned)(nb < 2048 ? nb : 2048);
+      hipLaunchKernelGGL((tile_fallback<ORD, MODE>), dim3(gf), b, 0, st, a, ntx, nty, ntz, nb, (const int *)fb);
+    }
+    if (own) (void)hipFreeAsync(fb, st);
   } else {
     hipLaunchKernelGGL((tile_fallback<ORD, MODE>), g, b, 0, st, a, ntx, nty, ntz, nb, (const int *)nullptr);
   }

@@ -68,3 +68,35 @@ def test_name_normalisation():
     assert inter_to_nitorch(3, "enum") == InterpolationType.cubic
     with pytest.raises(ValueError):
         bound_to_nitorch("sliding")
+
+
+def test_device_code_has_no_calls_needing_a_dynamic_stack(tmp_path):
+    """Every gfx950 kernel of the library has a static private segment: no
+    recursive call (e.g. a constexpr helper left unfolded) and no dynamic
+    stack, whose size the runtime cannot know (an overflow faults the GPU).
+    Reads the AMDGPU metadata notes of the library's own code objects."""
+    import shutil
+    import subprocess
+    from nitorch_amd import _lib
+    bundler = "/opt/rocm/llvm/bin/clang-offload-bundler"
+    readelf = "/opt/rocm/llvm/bin/llvm-readelf"
+    objcopy = shutil.which("objcopy")
+    if not (objcopy and os.path.exists(bundler) and os.path.exists(readelf)):
+        pytest.skip("binutils / ROCm llvm tools not available")
+    fat = tmp_path / "fat.bin"
+    # (-O binary with an output file: objcopy never rewrites the library)
+    subprocess.run([objcopy, "-O", "binary", "--only-section=.hip_fatbin", _lib.LIB_PATH, str(fat)], check=True)
+    data = fat.read_bytes()
+    magic = b"__CLANG_OFFLOAD_BUNDLE__"
+    starts = [m.start() for m in re.finditer(re.escape(magic), data)]
+    assert starts
+    kernels = 0
+    for i, s in enumerate(starts):
+        part, co = tmp_path / f"b{i}.bin", tmp_path / f"b{i}.co"
+        part.write_bytes(data[s:starts[i + 1] if i + 1 < len(starts) else len(data)])
+        subprocess.run([bundler, "--unbundle", "--type=o", f"--input={part}",
+                        "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--output={co}"], check=True)
+        notes = subprocess.run([readelf, "--notes", str(co)], check=True, capture_output=True, text=True).stdout
+        kernels += notes.count(".name:")
+        assert ".uses_dynamic_stack: true" not in notes, f"code object {i}: a kernel uses a dynamic stack"
+    assert kernels > 100
