@@ -203,6 +203,33 @@ def _shape(shape, default):
 # --------------------------------------------------------------------------
 # backend entry points
 # --------------------------------------------------------------------------
+def _half_as_float(fn):
+    """Half-precision tensors (the reference's GPU backend dispatches half,
+    nitorch/_C/src/impl/pushpull_common.cpp:2505-2537, with float as its
+    accumulation type) run the float32 kernels: float16 arguments are
+    widened to float32 and the results narrowed back to float16."""
+    import functools
+
+    @functools.wraps(fn)
+    def run(*args):
+        if not any(torch.is_tensor(t) and t.dtype == torch.float16 for t in args):
+            return fn(*args)
+        wide = []
+        for t in args:
+            if torch.is_tensor(t) and t.dtype == torch.float16:
+                w = t.float()
+                w.requires_grad_(t.requires_grad)
+                wide.append(w)
+            else:
+                wide.append(t)
+        out = fn(*wide)
+        if isinstance(out, list):
+            return [o.half() for o in out]
+        return out.half()
+    return run
+
+
+@_half_as_float
 def grid_pull(input, grid, bound, interpolation, extrapolate):
     dim = _check("grid_pull", input, grid)
     N = grid.shape[0]
@@ -218,6 +245,7 @@ def grid_pull(input, grid, bound, interpolation, extrapolate):
     return out
 
 
+@_half_as_float
 def grid_grad(input, grid, bound, interpolation, extrapolate):
     dim = _check("grid_grad", input, grid)
     N = grid.shape[0]
@@ -233,6 +261,7 @@ def grid_grad(input, grid, bound, interpolation, extrapolate):
     return out
 
 
+@_half_as_float
 def grid_push(input, grid, shape, bound, interpolation, extrapolate):
     dim = _check("grid_push", input, grid, target_compat=True)
     N, C = input.shape[:2]
@@ -249,6 +278,7 @@ def grid_push(input, grid, shape, bound, interpolation, extrapolate):
     return out
 
 
+@_half_as_float
 def grid_count(grid, shape, bound, interpolation, extrapolate):
     dim = _check("grid_count", None, grid)
     N = grid.shape[0]
@@ -265,6 +295,7 @@ def grid_count(grid, shape, bound, interpolation, extrapolate):
     return out
 
 
+@_half_as_float
 def grid_pull_backward(grad, input, grid, bound, interpolation, extrapolate):
     """Returns [grad_input if input.requires_grad] + [grad_grid if grid.requires_grad]
     (the reference's deque, pushpull.cpp:109-128)."""
@@ -292,6 +323,7 @@ def grid_pull_backward(grad, input, grid, bound, interpolation, extrapolate):
     return out
 
 
+@_half_as_float
 def grid_push_backward(grad, input, grid, bound, interpolation, extrapolate):
     dim = _check("grid_push_backward", input, grid)
     N, C = input.shape[:2]
@@ -315,6 +347,7 @@ def grid_push_backward(grad, input, grid, bound, interpolation, extrapolate):
     return out
 
 
+@_half_as_float
 def grid_count_backward(grad, grid, bound, interpolation, extrapolate):
     dim = _check("grid_count_backward", None, grid)
     N = grid.shape[0]
@@ -329,6 +362,7 @@ def grid_count_backward(grad, grid, bound, interpolation, extrapolate):
     return gg
 
 
+@_half_as_float
 def grid_grad_backward(grad, input, grid, bound, interpolation, extrapolate):
     dim = _check("grid_grad_backward", input, grid)
     N = grid.shape[0]
