@@ -86,6 +86,22 @@ int ni_grid_push(int32_t dtype, int32_t dim, const int32_t *bound, const int32_t
                  const void *grid, const int64_t *grid_stride, void *out,
                  const int64_t *out_stride, void *stream);
 
+/* No counterpart in the reference.  ni_grid_push for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): same arguments and result, but
+ * the list of scattered tiles that the tile path hands to its second
+ * kernel lives in a device-resident array of the library (2^20 ints per
+ * GPU, held until the process ends), so the call allocates nothing.
+ * Limit: N * ceil(X/4) * ceil(Y/8) * ceil(Z/16) <= 2^20 - 1 over in_size
+ * (X, Y, Z = 1 for missing dims); beyond it the call returns an error.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one
+ * GPU (concurrent streams, threads) race. */
+int ni_grid_push_noalloc(int32_t dtype, int32_t dim, const int32_t *bound, const int32_t *order,
+                         int32_t extrapolate, int64_t N, int64_t C, const int64_t *out_size,
+                         const int64_t *in_size, const void *input, const int64_t *input_stride,
+                         const void *grid, const int64_t *grid_stride, void *out,
+                         const int64_t *out_stride, void *stream);
+
 /* Replaces ni::grid_count (pushpull.h:34-38, pushpull.cpp:204-247).
  * grid [N, *in_size, dim] -> out [N, 1, *out_size] (contiguous). */
 int ni_grid_count(int32_t dtype, int32_t dim, const int32_t *bound, const int32_t *order,

@@ -24,6 +24,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 #include "pushpull_impl.h"
@@ -195,6 +196,30 @@ static int pushpull_entry(int dim, const int32_t *bound, const int32_t *order, i
   return run_pushpull<T>(a, (hipStream_t)stream);
 }
 
+// Device-resident scratch of ni_grid_push_noalloc: the tile path's list of
+// scattered tiles (a count, then tile ids), so that the call allocates
+// nothing.  Each GPU holds its own copy for the life of the process; its
+// address is looked up once per device.
+#define NI_SCRATCH_INTS (1 << 20)   // 4 MiB: up to 2^20 - 1 tiles
+static __device__ __align__(256) int scratch[NI_SCRATCH_INTS];
+static std::mutex g_scratch_mu;
+static int *g_scratch_ptr[64] = {nullptr};
+
+static int scratch_for_device(int **out) {
+  int dev = -1;
+  NI_HIP_CHECK(hipGetDevice(&dev));
+  NI_CHECK(dev >= 0 && dev < 64, "grid_push_noalloc: device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    NI_CHECK(rc == hipSuccess && p, "grid_push_noalloc: hipGetSymbolAddress failed (%s)", hipGetErrorString(rc));
+    g_scratch_ptr[dev] = (int *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 }  // namespace ni_amd
 
 using namespace ni_amd;
@@ -256,6 +281,35 @@ int ni_grid_push(int32_t dtype, int32_t dim, const int32_t *bound, const int32_t
   return ni_pushpull(dtype, dim, bound, order, extrapolate, F_PUSH | F_TRGT, N, C, out_size,
                      in_size, nullptr, nullptr, input, input_stride, grid, grid_stride, out,
                      out_stride, nullptr, nullptr, stream);
+}
+
+int ni_grid_push_noalloc(int32_t dtype, int32_t dim, const int32_t *bound, const int32_t *order,
+                         int32_t extrapolate, int64_t N, int64_t C, const int64_t *out_size,
+                         const int64_t *in_size, const void *input, const int64_t *input_stride,
+                         const void *grid, const int64_t *grid_stride, void *out,
+                         const int64_t *out_stride, void *stream) {
+  set_error("");
+  NI_CHECK(bound && order && out_size && in_size && grid && grid_stride, "grid_push_noalloc: missing argument");
+  NI_CHECK(dim >= 1 && dim <= 3, "grid_push_noalloc: dim must be 1, 2 or 3 (got %d)", dim);
+  // the most tiles the tile path can list: 4 x 8 x 16 samples per tile
+  int64_t tiles = N;
+  const int64_t per[3] = {4, 8, 16};
+  for (int k = 0; k < 3; ++k) tiles *= k < dim ? (in_size[k] + per[k] - 1) / per[k] : 1;
+  NI_CHECK(tiles + 1 <= NI_SCRATCH_INTS,
+           "grid_push_noalloc: %lld tiles of 4x8x16 samples exceed the scratch capacity of %d (at most %d "
+           "tiles; use ni_grid_push)", (long long)tiles, NI_SCRATCH_INTS, NI_SCRATCH_INTS - 1);
+  int *scr = nullptr;
+  if (int rc = scratch_for_device(&scr)) return rc;
+  if (dtype == NI_FLOAT32)
+    return pushpull_entry<float>(dim, bound, order, extrapolate, F_PUSH | F_TRGT, N, C, out_size, in_size, nullptr,
+                                 nullptr, input, input_stride, grid, grid_stride, out, out_stride, nullptr, nullptr,
+                                 stream, scr, NI_SCRATCH_INTS);
+  if (dtype == NI_FLOAT64)
+    return pushpull_entry<double>(dim, bound, order, extrapolate, F_PUSH | F_TRGT, N, C, out_size, in_size, nullptr,
+                                  nullptr, input, input_stride, grid, grid_stride, out, out_stride, nullptr, nullptr,
+                                  stream, scr, NI_SCRATCH_INTS);
+  set_error("grid_push_noalloc: unsupported dtype %d", dtype);
+  return 1;
 }
 
 int ni_grid_count(int32_t dtype, int32_t dim, const int32_t *bound, const int32_t *order,
