@@ -223,6 +223,16 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
+// p / d for 0 <= p, d < 2^22 from the float reciprocal rinv = 1 / d: the
+// float quotient is within 1 of the exact one, one correction each way
+// (a runtime integer division costs ~30 VALU operations)
+__device__ __forceinline__ int tl_div(int p, int d, float rinv) {
+  int q = (int)((float)p * rinv);
+  q -= q * d > p;
+  q += (q + 1) * d <= p;
+  return q;
+}
+
 // Rotate a small register array left by one: a rolled tap loop reads
 // element 0 and rotates, so that no array is indexed by the loop counter
 // (which would put it in scratch memory); NT rotations restore the order.
@@ -1035,6 +1045,7 @@ __global__ __launch_bounds__(DYN_NTH) void tile_dyn(PPArgs<float> a, int ntx, in
       continue;
     }
     const int VOL = ext[0] * DYZ;
+    const float rz = 1.f / (float)DZ, ry = 1.f / (float)ext[1];
     // folded offsets and signs of the box positions along each axis
     {
       const int64_t *st = GATHER ? a.src_s : a.out_s;
@@ -1065,7 +1076,8 @@ __global__ __launch_bounds__(DYN_NTH) void tile_dyn(PPArgs<float> a, int ntx, in
         if (c0) __syncthreads();
         // stage the folded, signed box of channels c0 .. c0 + nc
         for (int q = threadIdx.x; q < VOL; q += DYN_NTH) {
-          const int bx = q / DYZ, rr = q - bx * DYZ, by = rr / DZ, bz = rr - by * DZ;
+          const int row = tl_div(q, DZ, rz), bz = q - row * DZ;
+          const int bx = tl_div(row, ext[1], ry), by = row - bx * ext[1];
           const float sg = fsg[bx] * fsg[ext[0] + by] * fsg[ext[0] + ext[1] + bz];
           const int so = fofs[bx] + fofs[ext[0] + by] + fofs[ext[0] + ext[1] + bz];
 #pragma unroll
@@ -1131,7 +1143,8 @@ __global__ __launch_bounds__(DYN_NTH) void tile_dyn(PPArgs<float> a, int ntx, in
           const double dv = dbox[q];
           dbox[q] = 0.0;
           if (dv == 0.0) continue;
-          const int bx = q / DYZ, rr = q - bx * DYZ, by = rr / DZ, bz = rr - by * DZ;
+          const int row = tl_div(q, DZ, rz), bz = q - row * DZ;
+          const int bx = tl_div(row, ext[1], ry), by = row - bx * ext[1];
           const float sg = fsg[bx] * fsg[ext[0] + by] * fsg[ext[0] + ext[1] + bz];
           if (sg != 0.f)
             atomic_add(oc + fofs[bx] + fofs[ext[0] + by] + fofs[ext[0] + ext[1] + bz], (float)dv * sg);
@@ -1140,6 +1153,12 @@ __global__ __launch_bounds__(DYN_NTH) void tile_dyn(PPArgs<float> a, int ntx, in
     }
   }
 }
+
+}  // namespace ni_amd
+
+#include "pushpull_bin.h"   // the binned second pass (uses the helpers above)
+
+namespace ni_amd {
 
 // tile_kernel, then tile_fallback on the tiles it listed as too scattered
 // (list in stream-ordered scratch; without it, tile_fallback repeats the
@@ -1174,11 +1193,14 @@ static void launch_tile(const PPArgs<float> &a, hipStream_t st) {
   }
   if (fb) {
     if constexpr (MODE == TM_PULL || MODE == TM_PUSH || MODE == TM_COUNT) {
-      // listed tiles: sub-tiles with boxes sized to their spread
-      constexpr int nsub = (B::TX / DYN_TX) * (B::TY / DYN_TY) * (B::TZ / DYN_TZ);
-      const int64_t want = nb * nsub;
-      const unsigned gd = (unsigned)(want < 4096 ? want : 4096);
-      hipLaunchKernelGGL((tile_dyn<ORD, MODE>), dim3(gd), dim3(DYN_NTH), 0, st, a, ntx, nty, ntz, (const int *)fb);
+      // listed tiles: binned by first tap (when the call may allocate its
+      // bins), else sub-tiles with boxes sized to their spread
+      if (!own || !launch_binned<ORD, MODE>(a, ntx, nty, ntz, nb, fb, st)) {
+        constexpr int nsub = (B::TX / DYN_TX) * (B::TY / DYN_TY) * (B::TZ / DYN_TZ);
+        const int64_t want = nb * nsub;
+        const unsigned gd = (unsigned)(want < 4096 ? want : 4096);
+        hipLaunchKernelGGL((tile_dyn<ORD, MODE>), dim3(gd), dim3(DYN_NTH), 0, st, a, ntx, nty, ntz, (const int *)fb);
+      }
     } else {
       const unsigned gf = (unsigned)(nb < 2048 ? nb : 2048);
       hipLaunchKernelGGL((tile_fallback<ORD, MODE>), dim3(gf), b, 0, st, a, ntx, nty, ntz, nb, (const int *)fb);

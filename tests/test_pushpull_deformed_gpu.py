@@ -1,8 +1,9 @@
 """GPU parity of pull / push / count on strongly deformed grids, against the
 FP64 oracle: rotations with zoom (affine lattices) and smooth displacements
-with a large Jacobian.  Their tiles do not fit the static LDS box; they are
-cut into sub-tiles whose boxes are sized from their own spread (or, when
-still too large, run the per-sample routine).
+with a large Jacobian, and samples collapsed at random into a small
+region.  Their tiles do not fit the static LDS box; their samples are
+binned by first tap (a few bins of the last grid hold thousands of
+samples, processed in batches).
 
 Tolerances: pull 1e-5 norm-relative (order 0: bit-exact); push / count
 element-wise |gpu - oracle| <= 1e-4 * oracle(|v|) (+ 1e-36, below float's
@@ -50,6 +51,9 @@ def grids(rng, shape):
     yield "rot20z_zoom1.15", affine(shape, 20, 2, 1.15, (0.3, -0.7, 0.2))
     yield "rot35x_zoom0.8", affine(shape, 35, 0, 0.8, (1.1, 0.4, -0.5))
     yield "warp_amp6", warped(rng, shape, 6.0, 0.15)
+    # every sample scattered at random inside a 6^3 region: the tiles are
+    # too spread for the tile box and a few bins hold thousands of samples
+    yield "collapse6", (10 + 6 * rng.random((1,) + shape + (3,)))
 
 
 @pytest.mark.parametrize("order", [0, 1, 3, 5])
