@@ -131,30 +131,47 @@ __global__ __launch_bounds__(BIN_NTH) void bin_sort(PPArgs<float> a, int ntx, in
   }
 }
 
-// Exclusive scan of cnt[0, nb) into start[0, nb], then cnt = 0 (one
-// work-group of 1024 threads, each summing a contiguous run).
-static __global__ __launch_bounds__(1024) void bin_scan(int *cnt, int *start, int64_t nb) {
-  __shared__ int part[1024];
-  const int t = threadIdx.x;
-  const int64_t per = (nb + 1023) / 1024, b0 = t * per, b1 = b0 + per < nb ? b0 + per : nb;
-  int s = 0;
-  for (int64_t b = b0; b < b1; ++b) s += cnt[b];
-  part[t] = s;
+// Exclusive scan of cnt[0, nb) into start[0, nb], then cnt = 0: one
+// work-group of 1024 threads walks the bins in chunks of 4096 (4
+// consecutive bins a thread, coalesced), scanning each chunk by wave
+// shuffles and one pass over the 16 wave totals.  Returns at once when no
+// tile was listed (fb[0] == 0: nothing to sort).
+static __global__ __launch_bounds__(1024) void bin_scan(const int *fb, int *cnt, int *start, int64_t nb) {
+  __shared__ int wsum[16];
+  __shared__ int carry_s;
+  if (fb[0] == 0) return;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (t == 0) carry_s = 0;
   __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {   // inclusive Hillis-Steele scan
-    const int v = t >= o ? part[t - o] : 0;
+  for (int64_t c0 = 0; c0 < nb; c0 += 4096) {
+    const int64_t b = c0 + 4 * (int64_t)t;
+    int v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = b + k < nb ? cnt[b + k] : 0;
+    const int own = v[0] + v[1] + v[2] + v[3];
+    int inc = own;   // inclusive scan over the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += u;
+    }
+    if (lane == 63) wsum[wv] = inc;
     __syncthreads();
-    part[t] += v;
+    int before = carry_s;
+    for (int w = 0; w < wv; ++w) before += wsum[w];
+    int run = before + inc - own;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (b + k < nb) {
+        start[b + k] = run;
+        cnt[b + k] = 0;
+        run += v[k];
+      }
+    __syncthreads();   // every thread has read carry_s and wsum
+    if (t == 1023) carry_s = run;   // the chunk's end
     __syncthreads();
   }
-  int run = t ? part[t - 1] : 0;
-  for (int64_t b = b0; b < b1; ++b) {
-    const int c = cnt[b];
-    start[b] = run;
-    cnt[b] = 0;
-    run += c;
-  }
-  if (t == 1023) start[nb] = part[1023];
+  if (t == 0) start[nb] = carry_s;
 }
 
 // Folded offsets (source / output strides) and signs of one bin's box.
@@ -223,12 +240,13 @@ struct BinSamples {
 
 // Pull: one work-group per bin; 4 channels interleaved per box voxel.
 template <int ORD>
-__global__ __launch_bounds__(BIN_NTH) void bin_pull(PPArgs<float> a, int nbx, int nby, int nbz, const int *start,
-                                                    const int *idx, int64_t nbins) {
+__global__ __launch_bounds__(BIN_NTH) void bin_pull(PPArgs<float> a, int nbx, int nby, int nbz, const int *fb,
+                                                    const int *start, const int *idx, int64_t nbins) {
   constexpr int NT = ORD + 1, E = BinBox<ORD>::E, VOL = BinBox<ORD>::VOL;
   __shared__ __align__(16) float4 box[VOL];
   __shared__ int fofs[3 * E];
   __shared__ float fsg[3 * E];
+  if (fb[0] == 0) return;   // no listed tile: start[] was not written
   const int64_t b = xcd_block(nbins);
   const int s0 = start[b], cnt = start[b + 1] - s0;
   if (cnt == 0) return;
@@ -288,12 +306,13 @@ __global__ __launch_bounds__(BIN_NTH) void bin_pull(PPArgs<float> a, int nbx, in
 
 // Push / count: one work-group per bin; FP64 box, one channel at a time.
 template <int ORD, int MODE>
-__global__ __launch_bounds__(BIN_NTH, 4) void bin_push(PPArgs<float> a, int nbx, int nby, int nbz, const int *start,
-                                                    const int *idx, int64_t nbins) {
+__global__ __launch_bounds__(BIN_NTH, 4) void bin_push(PPArgs<float> a, int nbx, int nby, int nbz, const int *fb,
+                                                       const int *start, const int *idx, int64_t nbins) {
   constexpr int NT = ORD + 1, E = BinBox<ORD>::E, VOL = BinBox<ORD>::VOL;
   __shared__ double box[VOL];
   __shared__ int fofs[3 * E];
   __shared__ float fsg[3 * E];
+  if (fb[0] == 0) return;   // no listed tile: start[] was not written
   const int64_t b = xcd_block(nbins);
   const int s0 = start[b], cnt = start[b + 1] - s0;
   if (cnt == 0) return;
@@ -381,14 +400,14 @@ static bool launch_binned(const PPArgs<float> &a, int ntx, int nty, int ntz, int
   const unsigned gs = (unsigned)(gsamp < 8192 ? gsamp : 8192);
   hipLaunchKernelGGL((bin_sort<ORD, MODE, false>), dim3(gs), dim3(BIN_NTH), 0, st, a, ntx, nty, ntz, fb, nbx, nby,
                      nbz, cnt, (const int *)start, idx);
-  hipLaunchKernelGGL(bin_scan, dim3(1), dim3(1024), 0, st, cnt, start, nbins);
+  hipLaunchKernelGGL(bin_scan, dim3(1), dim3(1024), 0, st, fb, cnt, start, nbins);
   hipLaunchKernelGGL((bin_sort<ORD, MODE, true>), dim3(gs), dim3(BIN_NTH), 0, st, a, ntx, nty, ntz, fb, nbx, nby,
                      nbz, cnt, (const int *)start, idx);
   if constexpr (MODE == TM_PULL)
-    hipLaunchKernelGGL((bin_pull<ORD>), dim3((unsigned)nbins), dim3(BIN_NTH), 0, st, a, nbx, nby, nbz,
+    hipLaunchKernelGGL((bin_pull<ORD>), dim3((unsigned)nbins), dim3(BIN_NTH), 0, st, a, nbx, nby, nbz, fb,
                        (const int *)start, (const int *)idx, nbins);
   else
-    hipLaunchKernelGGL((bin_push<ORD, MODE>), dim3((unsigned)nbins), dim3(BIN_NTH), 0, st, a, nbx, nby, nbz,
+    hipLaunchKernelGGL((bin_push<ORD, MODE>), dim3((unsigned)nbins), dim3(BIN_NTH), 0, st, a, nbx, nby, nbz, fb,
                        (const int *)start, (const int *)idx, nbins);
   (void)hipFreeAsync(mem, st);
   return true;
