@@ -12,10 +12,12 @@
 // folded and signed with one FP32 global atomic per non-zero element.
 //
 // Counting sort in three launches over the listed samples: count per bin,
-// exclusive scan of the counts, scatter of the sample positions.  Samples
+// exclusive scan of the counts, scatter of the sample positions.  A pull
+// writes the zeros of its out-of-bound samples in the count pass; samples
 // whose first tap lies outside the binned range (far out-of-range
-// coordinates with extrapolation) and out-of-bound samples run the direct
-// per-sample routine pp_sample in the count pass.
+// coordinates with extrapolation) are listed there and run the direct
+// per-sample routine pp_sample in a kernel of their own (its registers
+// would otherwise weigh on the sort).
 //
 // Sample positions are packed as (x * Y + y) << 12 | z, which needs Z <=
 // 4096 and X * Y < 2^19; the host checks both.
@@ -107,14 +109,16 @@ __device__ __forceinline__ int wave_rank(int *cnt, int b) {
   return rank;
 }
 
-// Pass 1 (SCATTER = false): count the listed samples per bin and run the
-// unbinned ones directly.  Pass 2 (SCATTER = true): write each binned
-// sample's packed position at start[bin] + (its rank in the bin); `cnt`
-// was zeroed by the scan and counts again.
+// Pass 1 (SCATTER = false): count the listed samples per bin; a pull writes
+// the zeros of its out-of-bound samples, and in-bound samples whose first
+// tap lies outside the binned range are listed (listed-sample index, from
+// the end of idx, count in *rest) for bin_rest.  Pass 2 (SCATTER = true):
+// write each binned sample's packed position at start[bin] + (its rank in
+// the bin); `cnt` was zeroed by the scan and counts again.
 template <int ORD, int MODE, bool SCATTER>
 __global__ __launch_bounds__(BIN_NTH) void bin_sort(PPArgs<float> a, int ntx, int nty, int ntz, const int *fb,
                                                     int nbx, int nby, int nbz, int *cnt, const int *start,
-                                                    int *idx) {
+                                                    int *idx, int *rest, int nsamp) {
   using B = TileBox<ORD, !TileMeta<MODE>::gather>;
   constexpr int TS = B::TX * B::TY * B::TZ;
   const int64_t total = (int64_t)fb[0] * TS;
@@ -125,9 +129,28 @@ __global__ __launch_bounds__(BIN_NTH) void bin_sort(PPArgs<float> a, int ntx, in
     int lo[3];
     const bool ok = valid && bin_taps<ORD>(a, n, px, py, pz, xq, lo);
     const int b = ok ? (int)bin_of(lo, n, nbx, nby, nbz) : -1;
-    if (!SCATTER && valid && b < 0) pp_sample<float, int32_t, ORD, TileFlags<MODE>::value, true>(a, n, px, py, pz);
+    if (!SCATTER && MODE == TM_PULL && valid && !ok) {
+      float *op = a.out + (int64_t)n * a.out_s[0] + (int64_t)px * a.out_s[2] + (int64_t)py * a.out_s[3] +
+                  (int64_t)pz * a.out_s[4];
+      for (int c = 0; c < (int)a.C; ++c) op[(int64_t)c * a.out_s[1]] = 0.f;
+    }
+    if (!SCATTER && ok && b < 0) idx[nsamp - 1 - atomicAdd(rest, 1)] = (int)j;
     const int rank = wave_rank(cnt, b);
     if (SCATTER && b >= 0) idx[start[b] + rank] = ((px * (int)a.tY + py) << BIN_ZBITS) | pz;
+  }
+}
+
+// The listed samples that no bin holds, by the direct per-sample routine.
+template <int ORD, int MODE>
+__global__ __launch_bounds__(BIN_NTH) void bin_rest(PPArgs<float> a, int ntx, int nty, int ntz, const int *fb,
+                                                    const int *idx, const int *rest, int nsamp) {
+  using B = TileBox<ORD, !TileMeta<MODE>::gather>;
+  if (fb[0] == 0) return;
+  const int nr = *rest;
+  for (int r = blockIdx.x * BIN_NTH + threadIdx.x; r < nr; r += gridDim.x * BIN_NTH) {
+    int n, px, py, pz;
+    if (bin_sample<B>(a, fb, (int64_t)idx[nsamp - 1 - r], ntx, nty, ntz, n, px, py, pz))
+      pp_sample<float, int32_t, ORD, TileFlags<MODE>::value, true>(a, n, px, py, pz);
   }
 }
 
@@ -390,19 +413,22 @@ static bool launch_binned(const PPArgs<float> &a, int ntx, int nty, int ntz, int
   const int64_t nsamp = ntiles * TS;   // listed samples, at most
   if (nbins >= (int64_t(1) << 31) || nsamp >= (int64_t(1) << 31)) return false;
   int *mem = nullptr;
-  if (hipMallocAsync((void **)&mem, sizeof(int) * (size_t)(2 * nbins + 1 + nsamp), st) != hipSuccess) {
+  if (hipMallocAsync((void **)&mem, sizeof(int) * (size_t)(2 * nbins + 2 + nsamp), st) != hipSuccess) {
     (void)hipGetLastError();
     return false;
   }
-  int *cnt = mem, *start = mem + nbins, *idx = mem + 2 * nbins + 1;
+  // [rest count | cnt: nbins | start: nbins + 1 | idx: nsamp]
+  int *rest = mem, *cnt = mem + 1, *start = cnt + nbins, *idx = start + nbins + 1;
   const int64_t gz = (nbins + NI_BLOCK - 1) / NI_BLOCK, gsamp = (nsamp + BIN_NTH - 1) / BIN_NTH;
-  hipLaunchKernelGGL(fill_zero<int>, dim3((unsigned)(gz < 65536 ? gz : 65536)), dim3(NI_BLOCK), 0, st, cnt, nbins);
+  hipLaunchKernelGGL(fill_zero<int>, dim3((unsigned)(gz < 65536 ? gz : 65536)), dim3(NI_BLOCK), 0, st, mem, nbins + 1);
   const unsigned gs = (unsigned)(gsamp < 8192 ? gsamp : 8192);
   hipLaunchKernelGGL((bin_sort<ORD, MODE, false>), dim3(gs), dim3(BIN_NTH), 0, st, a, ntx, nty, ntz, fb, nbx, nby,
-                     nbz, cnt, (const int *)start, idx);
+                     nbz, cnt, (const int *)start, idx, rest, (int)nsamp);
   hipLaunchKernelGGL(bin_scan, dim3(1), dim3(1024), 0, st, fb, cnt, start, nbins);
   hipLaunchKernelGGL((bin_sort<ORD, MODE, true>), dim3(gs), dim3(BIN_NTH), 0, st, a, ntx, nty, ntz, fb, nbx, nby,
-                     nbz, cnt, (const int *)start, idx);
+                     nbz, cnt, (const int *)start, idx, rest, (int)nsamp);
+  hipLaunchKernelGGL((bin_rest<ORD, MODE>), dim3(gs < 1024 ? gs : 1024), dim3(BIN_NTH), 0, st, a, ntx, nty, ntz, fb,
+                     (const int *)idx, (const int *)rest, (int)nsamp);
   if constexpr (MODE == TM_PULL)
     hipLaunchKernelGGL((bin_pull<ORD>), dim3((unsigned)nbins), dim3(BIN_NTH), 0, st, a, nbx, nby, nbz, fb,
                        (const int *)start, (const int *)idx, nbins);

@@ -54,6 +54,15 @@ def grids(rng, shape):
     # every sample scattered at random inside a 6^3 region: the tiles are
     # too spread for the tile box and a few bins hold thousands of samples
     yield "collapse6", (10 + 6 * rng.random((1,) + shape + (3,)))
+    # the same with a tenth of the samples 40 voxels out along one axis:
+    # out of bounds, or (extrapolate=1) in bounds but beyond every bin,
+    # so they run the per-sample routine
+    g = 10 + 6 * rng.random((1,) + shape + (3,))
+    far = rng.random(shape) < 0.1
+    ax = rng.integers(0, 3, shape)
+    for d in range(3):
+        g[0][far & (ax == d), d] += rng.choice([-40.0, 40.0], int((far & (ax == d)).sum()))
+    yield "collapse6_far", g
 
 
 @pytest.mark.parametrize("order", [0, 1, 3, 5])
