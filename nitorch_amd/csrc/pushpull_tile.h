@@ -82,7 +82,11 @@ struct TileBox {
   static constexpr int DYZ = DY * DZ;
   static constexpr int DYZP = DYZ + ((48 - DYZ) % 64 + 64) % 64;
   static constexpr int VOL = DX * DYZP;
-  static constexpr int VOL4 = DX * DYZ;   // pull: 4 interleaved channels per voxel, unpadded
+  // pull: 4 interleaved channels (16 B) per voxel, x stride DYZ4 = 16 (mod
+  // 32) voxels, so that two x rows read in one LDS cycle fall on opposite
+  // halves of the banks
+  static constexpr int DYZ4 = DYZ + ((16 - DYZ) % 32 + 32) % 32;
+  static constexpr int VOL4 = DX * DYZ4;
   static constexpr int NPP = (DYZ + NTH - 1) / NTH;     // y-z plane positions per thread
   // channels staged at once: 1 for the scatters (a 52 KB box of doubles
   // at order 3) and 2 for pull / grid gradient, 4 for grid_grad (its 3 accumulators per
@@ -276,7 +280,7 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
   // read 16 consecutive 16-byte voxels, which no bank serves twice);
   // the other modes keep channel planes [c][x][y][z]
   constexpr bool IL = MODE == TM_PULL;
-  constexpr int DY = B::DY, DZ = B::DZ, DYZ = IL ? B::DYZ : B::DYZP, VOL = IL ? B::VOL4 : B::VOL;
+  constexpr int DY = B::DY, DZ = B::DZ, DYZ = IL ? B::DYZ4 : B::DYZP, VOL = IL ? B::VOL4 : B::VOL;
   static_assert(NCC >= 1 && NCC <= B::MAX_NCC, "channel chunk exceeds the LDS budget");
   static_assert(!IL || NCC == 4 || NCC == 1, "pull interleaves 4 channels (or runs one)");
   constexpr int NTH = B::NTH;
