@@ -14,8 +14,8 @@ pull on 2x3x128x160x128, both gradients; the cubic dct2 prefilter of the
 4x8x192^3 image; its cubic edge-aligned resize to 96^3), the Gauss-Newton
 solves of the 160x192x160 system and the SVF exponentiation and its analytic
 backward on that lattice, the order x bound sweep of pull/push/count/grad
-(orders 0/1/3/5 x dct2/dft/replicate/zero, one timed launch each after a
-warm-up), the roofline of the dominant kernel (algorithmic bytes per launch /
+(orders 0/1/3/5 x dct2/dft/replicate/zero, median of three timed launches
+each after a warm-up), the roofline of the dominant kernel (algorithmic bytes per launch /
 average launch time, HIP events on the launch stream), and the reference's
 own C++ CPU backend (oracle/_ref, built from the reference sources) timed on
 a bounded sample on the host cores.
@@ -40,7 +40,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-sweep", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--cpu-slices", type=int, default=192,
+    p.add_argument("--cpu-slices", type=int, default=48,
                    help="CPU baseline sample: first K x-slices of every batch element")
     p.add_argument("--no-solve", action="store_true")
     p.add_argument("--copy-calibration", action="store_true",
@@ -91,7 +91,9 @@ def main():
     S = g[..., 0].numel()           # spatial samples per op
     inb = ((g >= -0.05) & (g < torch.tensor(shape, device=dev, dtype=g.dtype) - 1 + 0.05)).all(-1)
     oob_frac = 1.0 - inb.float().mean().item()
-    y = torch.rand_like(x)          # values pushed
+    gen = torch.Generator().manual_seed(22)
+    y_cpu = torch.rand(x_cpu.shape, generator=gen)    # values pushed
+    y = y_cpu.to(dev)
 
     B3, O3 = ["dct2"], [3]
     pull = lambda: G.grid_pull(x, g, B3, O3, 0)
@@ -119,6 +121,33 @@ def main():
     }
     dom = max(ops, key=lambda k: ops[k]["ms"])
     achieved = gb_launch / (ops[dom]["ms"] / 1e3)
+
+    # the same pull / push on deformations far from the identity: the grid
+    # recipe with the displacement rescaled to its nominal std (8 voxels RMS),
+    # and an affine grid (15 degree rotation about y, zoom 1.1, about the
+    # centre); each with the fraction of tiles whose taps do not fit the LDS
+    # box (those run the per-sample kernel)
+    deformed = {}
+    for variant in ("rescaled", "affine"):
+        if variant == "rescaled":
+            gv = config2_inputs(rescale=True)[1].to(dev)
+        else:
+            gv = affine_grid_192(dev, N)
+        pv = lambda: G.grid_pull(x, gv, B3, O3, 0)
+        qv = lambda: G.grid_push(y, gv, shape, B3, O3, 0)
+        pms, _ = timed(pv, max(args.steps, 5), 1, stream)
+        qms, _ = timed(qv, max(args.steps, 5), 1, stream)
+        disp = (gv - g_ident(dev, shape)).double()
+        disp = disp[((gv >= 0) & (gv <= torch.tensor(shape, device=dev) - 1)).all(-1)]
+        for nm, ms, gather in (("pull", pms, True), ("push", qms, False)):
+            ops[f"{nm}_o3_dct2_{variant}"] = {"ms": ms, "gvox_s": S / ms / 1e6,
+                                              "hbm_frac": gb_launch / (ms / 1e3) / HBM_PEAK_GBS,
+                                              "fallback_tiles": fallback_fraction(gv, shape, 3, gather),
+                                              "displacement_rms": float(disp.pow(2).mean().sqrt())}
+        del gv
+    ops["pull_o3_dct2"]["fallback_tiles"] = fallback_fraction(g, shape, 3, True)
+    ops["push_o3_dct2"]["fallback_tiles"] = fallback_fraction(g, shape, 3, False)
+    ops["pull_o3_dct2"]["displacement_rms"] = float((g - g_ident(dev, shape)).double()[inb].pow(2).mean().sqrt())
 
     # backward_128: autograd backward of grid_pull (order 3, dft), both gradients
     from nitorch_amd.bench_data import config3_inputs
@@ -174,7 +203,8 @@ def main():
                     "grad": (lambda: G.grid_grad(x, g, Bb, Oo, 0), 12 * C if order == 0 else 12 + 4 * C + 12 * C),
                 }
                 for name, (fn, bps) in fns.items():
-                    ms, _ = timed(fn, 1, 1, stream)
+                    _, each = timed(fn, 3, 1, stream)
+                    ms = sorted(each)[1]
                     sweep[f"{name}_o{order}_{bound}"] = {
                         "ms": round(ms, 4), "gvox_s": round(S / ms / 1e6, 2),
                         "hbm_frac": round(bps * S / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
@@ -186,8 +216,18 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(x_cpu, g_cpu, args.cpu_slices)
-        cpu["pull_96_o1_dct2"] = cpu_pull96(x1c, g1c)
+        # GPU outputs on the CPU-baseline sample, for the error statements
+        k = args.cpu_slices
+        pull_out = pull().cpu()
+        ym = torch.zeros_like(y)
+        ym[:, :, :k] = y[:, :, :k]
+        push_slab = G.grid_push(ym, g, shape, B3, O3, 0).cpu()
+        ym.zero_()
+        ym[:1, :, :2] = y[:1, :, :2]
+        push_o = G.grid_push(ym, g, shape, B3, O3, 0)[:1].cpu()
+        del ym
+        cpu = cpu_baseline(x_cpu, g_cpu, y_cpu, pull_out, push_slab, push_o, k)
+        cpu["pull_96_o1_dct2"] = cpu_pull96(x1c, g1c, p1().cpu())
 
     line = {
         "metric": METRIC,
@@ -215,6 +255,63 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
+
+
+def g_ident(dev, shape):
+    import torch
+    mesh = torch.meshgrid(*[torch.arange(n, dtype=torch.float32, device=dev) for n in shape], indexing="ij")
+    return torch.stack(mesh, -1)
+
+
+def affine_grid_192(dev, batch, angle_deg=15.0, zoom=1.1, shape=(192, 192, 192)):
+    """Voxel coordinates of a rotation by `angle_deg` about the y axis and a
+    zoom, about the volume centre: x_in = zoom * R (x_out - c) + c."""
+    import math
+    import torch
+    a = math.radians(angle_deg)
+    rot = torch.tensor([[math.cos(a), 0.0, math.sin(a)], [0.0, 1.0, 0.0], [-math.sin(a), 0.0, math.cos(a)]],
+                       dtype=torch.float64, device=dev) * zoom
+    c = torch.tensor([(n - 1) / 2 for n in shape], dtype=torch.float64, device=dev)
+    ijk = g_ident(dev, shape).double() - c
+    out = (ijk @ rot.T + c).float()
+    return out.expand(batch, *out.shape).contiguous()
+
+
+def fallback_fraction(grid, shape, order, gather):
+    """Fraction of the tiles (4x8x16 samples for gathers, 8x16x16 for
+    scatters up to order 3) whose in-bound samples' first taps spread more
+    than the LDS box allows (tile extent + 1 per axis); those tiles take the
+    per-sample kernel."""
+    import torch
+    tile = (4, 8, 16) if (gather or order > 3) else (8, 16, 16)
+    N = grid.shape[0]
+    tol = 0.05
+    lim = torch.tensor(shape, device=grid.device, dtype=grid.dtype) - 1 + tol
+    inb = ((grid >= -tol) & (grid < lim)).all(-1)
+    if order == 0:
+        lo = torch.round(grid)
+    elif order == 1:
+        lo = torch.floor(grid)
+    else:
+        lo = torch.floor(grid - (order - 1) * 0.5)
+    pad = [(-n) % t for n, t in zip(grid.shape[1:4], tile)]
+    big = torch.tensor(2 ** 30, dtype=lo.dtype, device=lo.device)
+    fits = torch.ones((), dtype=torch.bool, device=grid.device)
+    nt = None
+    has = None
+    for d in range(3):
+        v = lo[..., d]
+        mn = torch.where(inb, v, big)
+        mx = torch.where(inb, v, -big)
+        mn = torch.nn.functional.pad(mn, (0, pad[2], 0, pad[1], 0, pad[0]), value=float(2 ** 30))
+        mx = torch.nn.functional.pad(mx, (0, pad[2], 0, pad[1], 0, pad[0]), value=-float(2 ** 30))
+        tx, ty, tz = (s // t for s, t in zip(mn.shape[1:], tile))
+        mn = mn.reshape(N, tx, tile[0], ty, tile[1], tz, tile[2]).amin((2, 4, 6))
+        mx = mx.reshape(N, tx, tile[0], ty, tile[1], tz, tile[2]).amax((2, 4, 6))
+        has = mn <= mx
+        fits = fits & (mx - mn <= tile[d] + 1)
+        nt = has.numel()
+    return round(float((has & ~fits).sum()) / nt, 4)
 
 
 def measured_traffic(op):
@@ -306,34 +403,83 @@ def svf_exp(dev, stream):
             "backward_ms": round(bms, 3), "backward": "exp_backward(rotate_grad=True), the anagrad path"}
 
 
-def cpu_baseline(x_cpu, g_cpu, slices):
+def host_facts():
+    """nproc, CPU model and torch's intra-op thread count of this host."""
+    import torch
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model, "torch_threads": torch.get_num_threads()}
+
+
+def _maxrel(a, b):
+    """max |a - b| / max |b| (norm-relative error)."""
+    import numpy as np
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.abs(a - b).max() / max(float(np.abs(b).max()), 1e-300))
+
+
+def _elementwise(a, b, bar):
+    """max over elements of |a - b| / bar (bar = the push of |values|)."""
+    import numpy as np
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    bar = np.asarray(bar, dtype=np.float64)
+    err = np.abs(a - b)
+    nz = bar > 0
+    if (err[~nz] > 0).any():
+        return float("inf")
+    return float((err[nz] / bar[nz]).max()) if nz.any() else 0.0
+
+
+def cpu_baseline(x_cpu, g_cpu, y_cpu, pull_gpu, push_masked, push_o, slices):
     """Reference C++ CPU backend (oracle/_ref, built from the reference
-    sources) on the first `slices` x-slices of every batch element (pull and
-    push, order 3, dct2; the default, all 192 slices, is the whole step, about
-    10 s on 16 host threads); falls back to the oracle port (single thread, a
-    few slices) when the reference build is absent."""
+    sources) on a bounded sample of the workload -- every batch element, the
+    first `slices` x-slices of the grid, pull and push, order 3, dct2, the
+    same input, grid and pushed values as the GPU -- timed as the median of 5
+    runs on all host threads, plus one single-thread run of a smaller
+    sample.  The GPU results are compared with the reference's on that
+    sample (pull: the GPU's full-size output restricted to the slab; push:
+    the GPU's push of the values zeroed outside the slab).  Falls back to
+    the oracle port (single thread, 4 slices of batch 0) when the reference
+    build is absent."""
+    import numpy as np
     import torch
     x = x_cpu.contiguous()
     g = g_cpu[:, :slices].contiguous()
+    y = y_cpu[:, :, :slices].contiguous()
     S = g[..., 0].numel()
     threads = torch.get_num_threads()
-    sample = (f"all {g.shape[0]} batch elements, first {slices} of 192 x-slices of the grid ({S} samples), "
-              f"8 channels, order 3, dct2, pull + push")
     try:
         from oracle import ref as R
         m = R.load()
     except Exception:
         m = None
+    errors = {}
     if m is not None:
         B, I = R.bounds(["dct2"]), R.orders([3])
-        y = torch.rand((g.shape[0], x.shape[1]) + tuple(g.shape[1:-1]))
-        t0 = time.perf_counter()
-        m.grid_pull(x, g, B, I, 0)
-        t1 = time.perf_counter()
-        m.grid_push(y, g, list(x.shape[2:]), B, I, 0)
-        t2 = time.perf_counter()
+        tp, tq = [], []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            rp = m.grid_pull(x, g, B, I, 0)
+            t1 = time.perf_counter()
+            rq = m.grid_push(y, g, list(x.shape[2:]), B, I, 0)
+            t2 = time.perf_counter()
+            tp.append(t1 - t0)
+            tq.append(t2 - t1)
+        pull_s, push_s = sorted(tp)[2], sorted(tq)[2]
+        errors["pull_vs_reference"] = _maxrel(pull_gpu[:, :, :slices].double().numpy(), rp.double().numpy())
+        errors["push_vs_reference"] = _maxrel(push_masked.double().numpy(), rq.double().numpy())
         kind = "reference"
-        # the same on one host thread, batch 0, first 8 x-slices
+        sample = (f"all {g.shape[0]} batch elements, first {slices} of {g_cpu.shape[1]} x-slices of the grid "
+                  f"({S} samples), 8 channels, order 3, dct2, pull + push; median of 5 runs")
         g1 = g[:1, :8].contiguous()
         y1 = y[:1, :, :8].contiguous()
         S1 = g1[..., 0].numel()
@@ -346,28 +492,43 @@ def cpu_baseline(x_cpu, g_cpu, slices):
         finally:
             torch.set_num_threads(threads)
         single = {"value": round(2 * S1 / (u1 - u0) / 1e9, 6), "unit": "Gvox/s", "cores": 1,
-                  "sample": f"batch 0, first 8 x-slices ({S1} samples), pull + push"}
+                  "sample": f"batch 0, first 8 x-slices ({S1} samples), pull + push, one run"}
     else:
         from oracle import pushpull as O
-        x = x[:1]
-        g = g[:1, :4]
+        x, g, y = x[:1], g[:1, :4], y[:1, :, :4]
         S = g[..., 0].numel()
-        y = torch.rand((1, x.shape[1]) + tuple(g.shape[1:-1])).numpy()
         t0 = time.perf_counter()
         O.grid_pull(x.numpy(), g.numpy(), "dct2", 3, 0)
         t1 = time.perf_counter()
-        O.grid_push(y, g.numpy(), x.shape[2:], "dct2", 3, 0)
+        O.grid_push(y.numpy(), g.numpy(), x.shape[2:], "dct2", 3, 0)
         t2 = time.perf_counter()
+        pull_s, push_s = t1 - t0, t2 - t1
         kind, threads = "port", 1
-        sample = f"batch 0, first 4 x-slices ({S} samples), single-threaded oracle port"
+        sample = f"batch 0, first 4 x-slices ({S} samples), single-threaded oracle port, one run"
         single = None
-    return {"value": round(2 * S / (t2 - t0) / 1e9, 5), "unit": "Gvox/s", "cores": threads,
-            "kind": kind, "sample": sample,
-            "pull_s": round(t1 - t0, 3), "push_s": round(t2 - t1, 3), "single_thread": single}
+    # error against the FP64 oracle: batch 0, first 2 x-slices
+    from oracle import pushpull as O
+    k = min(2, slices)
+    x64 = x_cpu[:1].double().numpy()
+    g64 = g_cpu[:1, :k].double().numpy()
+    y64 = y_cpu[:1, :, :k].double().numpy()
+    errors["pull_vs_fp64_oracle"] = _maxrel(pull_gpu[:1, :, :k].double().numpy(), O.grid_pull(x64, g64, "dct2", 3, 0))
+    shp = tuple(x_cpu.shape[2:])
+    errors["push_vs_fp64_oracle_elementwise"] = _elementwise(push_o.double().numpy(),
+                                                             O.grid_push(y64, g64, shp, "dct2", 3, 0),
+                                                             O.grid_push(abs(y64), g64, shp, "dct2", 3, 0))
+    errors["definition"] = ("pull: max|gpu - ref| / max|ref| over the sample; push vs reference: the same "
+                            "norm-relative error; push vs FP64 oracle: max over elements of |gpu - ref| / "
+                            "push(|values|) (batch 0, first 2 x-slices)")
+    return {"value": round(2 * S / (pull_s + push_s) / 1e9, 5), "unit": "Gvox/s", "cores": threads,
+            "kind": kind, "sample": sample, "host": host_facts(),
+            "pull_s": round(pull_s, 3), "push_s": round(push_s, 3), "single_thread": single,
+            "errors": errors}
 
 
-def cpu_pull96(x, g):
-    """The reference CPU backend on the whole pull_96 case (median of 5)."""
+def cpu_pull96(x, g, gpu_out):
+    """The reference CPU backend on the whole pull_96 case (median of 5),
+    and the GPU output's error against it."""
     import torch
     try:
         from oracle import ref as R
@@ -377,7 +538,7 @@ def cpu_pull96(x, g):
     if m is None:
         return None
     B, I = R.bounds(["dct2"]), R.orders([1])
-    m.grid_pull(x, g, B, I, 0)
+    ref = m.grid_pull(x, g, B, I, 0)
     ts = []
     for _ in range(5):
         t0 = time.perf_counter()
@@ -385,7 +546,7 @@ def cpu_pull96(x, g):
         ts.append(time.perf_counter() - t0)
     ms = sorted(ts)[2] * 1e3
     return {"ms": round(ms, 3), "gvox_s": round(g[..., 0].numel() / ms / 1e6, 4), "cores": torch.get_num_threads(),
-            "kind": "reference"}
+            "kind": "reference", "error_vs_reference": _maxrel(gpu_out.double().numpy(), ref.double().numpy())}
 
 
 if __name__ == "__main__":
