@@ -126,7 +126,7 @@ def main():
     # recipe with the displacement rescaled to its nominal std (8 voxels RMS),
     # and an affine grid (15 degree rotation about y, zoom 1.1, about the
     # centre); each with the fraction of tiles whose taps do not fit the LDS
-    # box (those run the per-sample kernel)
+    # box (their samples are sorted by first-tap bin and run the binned pass)
     deformed = {}
     for variant in ("rescaled", "affine"):
         if variant == "rescaled":
@@ -280,8 +280,8 @@ def affine_grid_192(dev, batch, angle_deg=15.0, zoom=1.1, shape=(192, 192, 192))
 def fallback_fraction(grid, shape, order, gather):
     """Fraction of the tiles (4x8x16 samples for gathers, 8x16x16 for
     scatters up to order 3) whose in-bound samples' first taps spread more
-    than the LDS box allows (tile extent + 1 per axis); those tiles take the
-    per-sample kernel."""
+    than the LDS box allows (tile extent + 1 per axis); those tiles' samples
+    are sorted by first-tap bin and run the binned second pass."""
     import torch
     tile = (4, 8, 16) if (gather or order > 3) else (8, 16, 16)
     N = grid.shape[0]
