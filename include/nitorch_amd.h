@@ -208,6 +208,16 @@ int ni_relax_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *he
                   const void *weight, const void *gradient, void *solution, const int32_t *bound,
                   const double *voxel_size, const double *penalty, int32_t nb_iter, void *stream);
 
+/* The colour classes ni_relax_grid and the relax smoother of ni_fmg_grid /
+ * ni_fmg_solve use on a lattice (3-D shape, bounds, penalties as there):
+ * plan[0] = 1 for a linear colouring (a x + b y + c z) mod m, 0 for per-axis
+ * period classes; plan[1] = the number of classes (m); plan[2..4] = a, b, c
+ * (0 for per-axis classes).  Host only, no device work.  No counterpart in the
+ * reference, whose colour classes are fixed per-axis periods
+ * (relax_grid_common.cpp:454-476); for callers that need the relaxation
+ * order, such as tests that restate it. */
+int ni_relax_colouring(const int64_t *shape, const int32_t *bound, const double *penalty, int32_t *plan);
+
 /* Replace ni::fmg_restriction / ni::fmg_prolongation (nitorch/_C/src/multires.h):
  * [N, C, *in_shape] -> [N, C, *out_shape] (linear-adjoint restriction,
  * quadratic prolongation). */

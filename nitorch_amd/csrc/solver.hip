@@ -26,11 +26,12 @@
 //    voxel (3 components x 25 positions) are shared between the own-component
 //    and the Lame cross terms; boundary folding is computed per axis (5
 //    offsets) and only branches at the border;
-//  * Gauss-Seidel uses 3x3x3 colour classes (bending, radius-2 stencil),
-//    2x2x2 (Lame only) or red/black (membrane only), with extra classes for
-//    the remainder slices of circulant axes: each class is an independent set,
-//    so each launch is a race-free parallel update and the sweep is a true
-//    multi-colour Gauss-Seidel;
+//  * Gauss-Seidel visits colour classes that are independent sets of the
+//    stencil, one launch per class: a linear colouring (a x + b y + c z) mod m
+//    where one exists (8 classes for bending on circulant lattices that 8
+//    divides, 7 with reflecting bounds, red/black for membrane only), else
+//    per-axis period classes (3x3x3 for bending, 2x2x2 otherwise) with extra
+//    classes for the remainder slices of circulant axes;
 //  * levels with few voxels are relaxed by a single work-group that runs all
 //    sweeps and colours with barriers in between (one launch per relax);
 //  * restriction / prolongation gather (4^3 / 3^3 taps) with the per-channel
@@ -38,6 +39,8 @@
 //  * the FMG driver runs on the host, enqueuing on one stream, with all
 //    pyramid storage carved from a caller-provided workspace (no allocation
 //    during a solve).
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <cmath>
 #include <vector>
@@ -394,32 +397,66 @@ __device__ __forceinline__ void relax_voxel(T *__restrict__ xn, const T *__restr
   xn[2 * V + i] = x2 + idt * (r0 * (p * s - q * b) + r1 * (p * q - a * s) + r2 * (a * b - p * p));
 }
 
-// Colour classes of a multi-colour sweep.  Along each axis a class is
-// {start + step * i, i < count}; a sweep visits every (x, y, z) class triple.
-// With period F (3 for bending's radius-2 stencil, 2 for Lame only) voxels of
-// a class are never coupled, so a class is updated in parallel without races.
-// Circulant axes whose length is not a multiple of F get one extra class per
-// remainder slice (otherwise the wrap-around couples two voxels of a class).
+// Colour classes of a multi-colour sweep.  Each class is an independent set
+// of the stencil (no two of its voxels are coupled), so a class is updated
+// in parallel without races and the sweep is a true multi-colour
+// Gauss-Seidel.  Two kinds of plan:
+//  * linear: voxel (x, y, z) has colour (a x + b y + c z) mod m.  Every
+//    coupling offset v of the stencil (bending: |v|_1 <= 2; Lame only:
+//    |v|_1 <= 2, |v|_inf <= 1; membrane only: |v|_1 = 1) must have
+//    a v0 + b v1 + c v2 != 0 (mod m), with v reduced modulo the length of a
+//    circulant axis, whose coefficient * length must be 0 (mod m) so that the
+//    colour is defined on the torus (folds of the other bounds never take a
+//    neighbour farther away).  The 3-D bending stencil needs m >= 7 (the
+//    |v|_1 <= 1 balls around a class's voxels are disjoint): 7 with
+//    reflecting bounds, 8 on circulant lattices whose sizes 8 divides, where
+//    a 3x3x3 product needs 27 classes or more;
+//  * per-axis: along each axis a class is {start + step * i, i < count} with
+//    period F (3 for bending's radius-2 stencil, 2 for Lame / membrane),
+//    with one extra class per remainder slice of a circulant axis whose
+//    length F does not divide; a sweep visits every class triple.
+// The linear plan is used when one exists with m <= 32 classes and no more
+// than the per-axis plan has.
 struct AxisColours {
   int n;
   int start[8], step[8], count[8];
 };
 
+struct LinColour {
+  int m, a, b, c;
+  int gz, s, cinv;         // gcd(c, m), m / gz, inverse of c / gz modulo s
+};
+
 struct ColourPlan {
-  int redblack;            // 1: checkerboard (x + y + z) % 2 classes
+  int lin;                 // 1: linear colouring L, else per-axis classes ax
+  LinColour L;
   AxisColours ax[3];
 };
 
-// One class triple (or one red/black class) of a sweep.
+// z of the k-th member of linear class col in row (x, y), or -1: the members
+// solve c z = col - a x - b y (mod m), i.e. z = z0 (mod s) when gz divides
+// the right-hand side, and there are none otherwise.
+__device__ __forceinline__ int lin_member_z(const LinColour &L, int col, int xx, int y, int k, int Z) {
+  int rhs = (col - L.a * xx - L.b * y) % L.m;
+  if (rhs < 0) rhs += L.m;
+  if (rhs % L.gz) return -1;
+  const int z = (rhs / L.gz) * L.cinv % L.s + L.s * k;
+  return z < Z ? z : -1;
+}
+
+// One class of a sweep: linear class `col`, or the per-axis class triple
+// (start, step, count per axis).  Linear classes: one thread per slot k of a
+// row (x, y), z = z0 + s k, so consecutive lanes take consecutive members of
+// a row.
 template <typename T>
 __global__ __launch_bounds__(SV_BLOCK) void relax_colour(T *__restrict__ x, const T *__restrict__ g, const T *__restrict__ h,
-                                                         int CC, Geo G, StencilT<T> S, int N, int redblack, int rb,
-                                                         int sx, int tx, int nx, int sy, int ty, int ny,
+                                                         int CC, Geo G, StencilT<T> S, int N, int lin, LinColour L,
+                                                         int col, int sx, int tx, int nx, int sy, int ty, int ny,
                                                          int sz, int tz, int nz) {
   const int V = G.X * G.Y * G.Z;
   const long long gid = sv_block() * SV_BLOCK + threadIdx.x;
   int n, xx, y, z;
-  if (!redblack) {
+  if (!lin) {
     const long long per = (long long)nx * ny * nz;
     if (gid >= per * N) return;
     n = (int)(gid / per);
@@ -428,26 +465,23 @@ __global__ __launch_bounds__(SV_BLOCK) void relax_colour(T *__restrict__ x, cons
     y = sy + ty * ((j / nz) % ny);
     xx = sx + tx * (j / (nz * ny));
   } else {
-    // red/black: class rb, half of each row along z
-    const int nzh = (G.Z + 1) / 2;
-    const long long per = (long long)G.X * G.Y * nzh;
+    const int K = (G.Z + L.s - 1) / L.s;
+    const long long per = (long long)G.X * G.Y * K;
     if (gid >= per * N) return;
     n = (int)(gid / per);
-    const int j = (int)(gid % per);
-    const int row = j / nzh;
+    const long long j = gid % per;
+    const int row = (int)(j / K), k = (int)(j % K);
     y = row % G.Y;
     xx = row / G.Y;
-    z = (j % nzh) * 2 + ((xx + y + rb) & 1);
-    if (z >= G.Z) return;
+    z = lin_member_z(L, col, xx, y, k, G.Z);
+    if (z < 0) return;
   }
   relax_voxel<T>(x + (size_t)n * 3 * V, g + (size_t)n * 3 * V, h ? h + (size_t)n * CC * V : nullptr,
                  CC, V, G, S, n, xx, y, z);
 }
 
-// All sweeps of a small level in one work-group, with a barrier between
-// classes; threads enumerate the members of each class directly.
-// Small levels: one work-group runs every sweep and colour with barriers in
-// between, on a copy of the solution held in LDS (3 N V values, up to the
+// Small levels: one work-group runs every sweep and colour class with
+// barriers in between (threads enumerate the members of each class), on a copy of the solution held in LDS (3 N V values, up to the
 // device's LDS per work-group), so the 75 stencil reads per voxel and the
 // updates never leave the CU.  32-bit index arithmetic (these levels are small).
 // Double precision runs 512 threads, so that its stencil state fits the
@@ -464,15 +498,15 @@ __global__ __launch_bounds__(relax_small_threads<T>()) void relax_small(T *__res
   for (int i = threadIdx.x; i < 3 * N * V; i += blockDim.x) xs[i] = x[i];
   __syncthreads();
   for (int it = 0; it < nb_iter; ++it) {
-    if (P.redblack) {
-      for (int rb = 0; rb < 2; ++rb) {
-        const int nzh = (G.Z + 1) / 2;
-        const int per = G.X * G.Y * nzh;
+    if (P.lin) {
+      const int K = (G.Z + P.L.s - 1) / P.L.s;
+      const int per = G.X * G.Y * K;
+      for (int col = 0; col < P.L.m; ++col) {
         for (int k = threadIdx.x; k < per * N; k += blockDim.x) {
           const int n = k / per, j = k - n * per;
-          const int row = j / nzh, y = row % G.Y, xx = row / G.Y;
-          const int z = (j - row * nzh) * 2 + ((xx + y + rb) & 1);
-          if (z < G.Z)
+          const int row = j / K, y = row % G.Y, xx = row / G.Y;
+          const int z = lin_member_z(P.L, col, xx, y, j - row * K, G.Z);
+          if (z >= 0)
             relax_voxel<T>(xs + (size_t)n * 3 * V, g + (size_t)n * 3 * V, h ? h + (size_t)n * CC * V : nullptr,
                            CC, V, G, S, n, xx, y, z);
         }
@@ -960,19 +994,84 @@ static void axis_colours(AxisColours &c, int n, int F, int bound) {
   }
 }
 
+static int gcd_int(int u, int v) {
+  while (v) { const int t = u % v; u = v; v = t; }
+  return u;
+}
+
+static int mod_pos(long long a, int m) {
+  const long long r = a % m;
+  return (int)(r < 0 ? r + m : r);
+}
+
+// Smallest linear colouring (m <= 32) of the lattice for the stencil band,
+// ties broken by the smallest gcd(c, m), then c, a, b; false if none.
+static bool lin_colouring(const Geo &G, int band, LinColour &L) {
+  const int sz[3] = {G.X, G.Y, G.Z};
+  int off[124][3], no = 0;
+  for (int i = -2; i <= 2; ++i)
+    for (int j = -2; j <= 2; ++j)
+      for (int k = -2; k <= 2; ++k) {
+        int v[3] = {i, j, k};
+        const int l1 = std::abs(i) + std::abs(j) + std::abs(k);
+        const int li = std::max(std::abs(i), std::max(std::abs(j), std::abs(k)));
+        const bool coupled = band == 3 ? l1 <= 2 : band == 2 ? (l1 <= 2 && li <= 1) : band == 0 ? l1 == 1 : false;
+        if (!l1 || !coupled) continue;
+        bool ok = true, nz = false;
+        for (int d = 0; d < 3; ++d) {
+          const int n = sz[d];
+          if (n == 1) v[d] = 0;
+          else if (G.b[d] == B_DFT) { int r = mod_pos(v[d], n); if (r > n / 2) r -= n; v[d] = r; }
+          else if (std::abs(v[d]) > n - 1) ok = false;
+          nz = nz || v[d] != 0;
+        }
+        if (ok && nz) { off[no][0] = v[0]; off[no][1] = v[1]; off[no][2] = v[2]; ++no; }
+      }
+  for (int m = 1; m <= 32; ++m) {
+    bool found = false;
+    int best_g = 0;
+    for (int c = 0; c < m; ++c) {
+      const int gz = gcd_int(c, m);
+      if (found && gz >= best_g) continue;
+      for (int a = 0; a < m; ++a)
+        for (int b = 0; b < m; ++b) {
+          const int co[3] = {a, b, c};
+          bool ok = true;
+          for (int d = 0; d < 3 && ok; ++d)
+            if (G.b[d] == B_DFT && sz[d] > 1 && ((long long)co[d] * sz[d]) % m) ok = false;
+          for (int o = 0; o < no && ok; ++o)
+            if (mod_pos((long long)a * off[o][0] + (long long)b * off[o][1] + (long long)c * off[o][2], m) == 0)
+              ok = false;
+          if (ok && (!found || gz < best_g)) {
+            found = true; best_g = gz;
+            L.m = m; L.a = a; L.b = b; L.c = c;
+          }
+        }
+    }
+    if (found) {
+      L.gz = gcd_int(L.c, L.m);
+      L.s = L.m / L.gz;
+      L.cinv = 0;
+      const int cr = (L.c / L.gz) % L.s;
+      for (int t = 0; t < L.s; ++t)
+        if ((cr * t) % L.s == 1 % L.s) { L.cinv = t; break; }
+      return true;
+    }
+  }
+  return false;
+}
+
 static ColourPlan make_plan(const Geo &G, int band) {
   ColourPlan P;
   std::memset(&P, 0, sizeof(P));
   const int sz[3] = {G.X, G.Y, G.Z};
-  if (band == 0) {
-    // red/black unless a circulant axis has odd length (then 2x2x2 classes)
-    bool ok = true;
-    for (int d = 0; d < 3; ++d)
-      if (G.b[d] == B_DFT && (sz[d] & 1) && sz[d] > 1) ok = false;
-    if (ok) { P.redblack = 1; return P; }
-    band = 2;
+  const int F = band == 3 ? 3 : (band == 1 ? 1 : 2);
+  for (int d = 0; d < 3; ++d) axis_colours(P.ax[d], sz[d], F, G.b[d]);
+  LinColour L;
+  if (lin_colouring(G, band, L) && L.m <= P.ax[0].n * P.ax[1].n * P.ax[2].n) {
+    P.lin = 1;
+    P.L = L;
   }
-  for (int d = 0; d < 3; ++d) axis_colours(P.ax[d], sz[d], band, G.b[d]);
   return P;
 }
 
@@ -998,21 +1097,19 @@ static size_t small_lds_bytes() {
 
 template <typename T>
 static void launch_relax(T *x, const T *g, const T *h, int CC, const Geo &G, const StencilT<T> &S,
-                         int N, int band, int nb_iter, hipStream_t st) {
+                         int N, const ColourPlan &P, int nb_iter, hipStream_t st) {
   const long long V = (long long)G.X * G.Y * G.Z;
-  const ColourPlan P = make_plan(G, band);
   const size_t xbytes = (size_t)3 * N * V * sizeof(T);
   if (xbytes <= small_lds_bytes<T>()) {
     hipLaunchKernelGGL(relax_small<T>, dim3(1), dim3(relax_small_threads<T>()), xbytes, st, x, g, h, CC, G, S, N, P, nb_iter);
     return;
   }
   for (int it = 0; it < nb_iter; ++it) {
-    if (P.redblack) {
-      for (int rb = 0; rb < 2; ++rb) {
-        const long long n = (long long)N * G.X * G.Y * ((G.Z + 1) / 2);
-        hipLaunchKernelGGL(relax_colour<T>, dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, x, g, h, CC, G, S, N, 1, rb,
-                           0, 0, 0, 0, 0, 0, 0, 0, 0);
-      }
+    if (P.lin) {
+      const long long n = (long long)N * G.X * G.Y * ((G.Z + P.L.s - 1) / P.L.s);
+      for (int col = 0; col < P.L.m; ++col)
+        hipLaunchKernelGGL(relax_colour<T>, dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, x, g, h, CC, G, S, N, 1, P.L,
+                           col, 0, 0, 0, 0, 0, 0, 0, 0, 0);
       continue;
     }
     for (int a = 0; a < P.ax[0].n; ++a)
@@ -1021,8 +1118,8 @@ static void launch_relax(T *x, const T *g, const T *h, int CC, const Geo &G, con
           const AxisColours &ax = P.ax[0], &ay = P.ax[1], &az = P.ax[2];
           const long long n = (long long)N * ax.count[a] * ay.count[b] * az.count[c];
           if (n == 0) continue;
-          hipLaunchKernelGGL(relax_colour<T>, dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, x, g, h, CC, G, S, N, 0, 0,
-                             ax.start[a], ax.step[a], ax.count[a], ay.start[b], ay.step[b], ay.count[b],
+          hipLaunchKernelGGL(relax_colour<T>, dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, x, g, h, CC, G, S, N, 0, P.L,
+                             0, ax.start[a], ax.step[a], ax.count[a], ay.start[b], ay.step[b], ay.count[b],
                              az.start[c], az.step[c], az.count[c]);
         }
   }
@@ -1169,6 +1266,7 @@ struct FMG {
   T *g[16], *x[16], *r[16];
   StencilT<T> S[16];
   Geo G[16];
+  ColourPlan P[16];        // Gauss-Seidel colour classes per level
   int bound[3];
   T *cg_z, *cg_p;          // PCG smoother scratch (sized for level 0)
   double *cg_sc;
@@ -1184,7 +1282,7 @@ struct FMG {
     if (use_cg)
       run_pcg<T>(x[l], gl, h[l], CC, G[l], S[l], N, nb_iter, 0.0, r[l], cg_z, cg_p, cg_sc, st, nullptr);
     else
-      launch_relax<T>(x[l], gl, h[l], CC, G[l], S[l], N, band, nb_iter, st);
+      launch_relax<T>(x[l], gl, h[l], CC, G[l], S[l], N, P[l], nb_iter, st);
   }
   void restrict_field(const T *fine, T *coarse, int l) {   // level l -> l+1, displacement scaling
     float f[3], cs[6] = {1, 1, 1, 1, 1, 1};
@@ -1267,6 +1365,7 @@ static int fmg_setup(FMG<T> &F, int N, const int *sz, const T *hess, int CC, con
     F.S[l] = cast_weights<T>(make_weights(F.L[l].vs, pen), F.w[l]);
     F.G[l].X = F.L[l].sz[0]; F.G[l].Y = F.L[l].sz[1]; F.G[l].Z = F.L[l].sz[2];
     for (int d = 0; d < 3; ++d) F.G[l].b[d] = bound[d];
+    if (!use_cg) F.P[l] = make_plan(F.G[l], F.band);
   }
   // pyramid of Hessians, gradients and initial solutions
   for (int l = 1; l < F.NL; ++l) {
@@ -1375,16 +1474,32 @@ int ni_relax_grid(int32_t dtype, int64_t N, const int64_t *shape, const void *he
   G.X = sz[0]; G.Y = sz[1]; G.Z = sz[2];
   for (int d = 0; d < 3; ++d) G.b[d] = bound[d];
   const StencilW W = make_weights(voxel_size, penalty);
-  const int band = relax_band(penalty);
+  const ColourPlan P = make_plan(G, relax_band(penalty));
   hipStream_t st = (hipStream_t)stream;
   if (dtype == NI_FLOAT32)
     launch_relax<float>((float *)solution, (const float *)gradient, (const float *)hessian, hessian ? CC : 0, G,
-                        cast_weights<float>(W, (const float *)weight), (int)N, band, nb_iter, st);
+                        cast_weights<float>(W, (const float *)weight), (int)N, P, nb_iter, st);
   else if (dtype == NI_FLOAT64)
     launch_relax<double>((double *)solution, (const double *)gradient, (const double *)hessian, hessian ? CC : 0, G,
-                         cast_weights<double>(W, (const double *)weight), (int)N, band, nb_iter, st);
+                         cast_weights<double>(W, (const double *)weight), (int)N, P, nb_iter, st);
   else NI_CHECK(false, "relax_grid: unsupported dtype %d", dtype);
   NI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int ni_relax_colouring(const int64_t *shape, const int32_t *bound, const double *penalty, int32_t *plan) {
+  set_error("");
+  NI_CHECK(shape && bound && penalty && plan, "relax_colouring: missing argument");
+  Geo G;
+  G.X = (int)shape[0]; G.Y = (int)shape[1]; G.Z = (int)shape[2];
+  NI_CHECK(G.X > 0 && G.Y > 0 && G.Z > 0, "relax_colouring: empty lattice");
+  for (int d = 0; d < 3; ++d) G.b[d] = bound[d];
+  const ColourPlan P = make_plan(G, relax_band(penalty));
+  plan[0] = P.lin;
+  plan[1] = P.lin ? P.L.m : P.ax[0].n * P.ax[1].n * P.ax[2].n;
+  plan[2] = P.lin ? P.L.a : 0;
+  plan[3] = P.lin ? P.L.b : 0;
+  plan[4] = P.lin ? P.L.c : 0;
   return 0;
 }
 

@@ -147,6 +147,21 @@ def residual_grid_cf(x, g, hessian=None, penalty=(0, 0, 0, 0, 0), voxel_size=(1,
     return out
 
 
+def relax_colouring(shape, bound="dft", penalty=(0, 0, 0, 0, 0)):
+    """Colour classes of the Gauss-Seidel sweeps on a 3-D lattice (host
+    only): ("linear", m, (a, b, c)) -- colour (a x + b y + c z) mod m,
+    classes 0 .. m-1 in turn -- or ("axes", n_classes, None) for per-axis
+    period classes."""
+    fn = L.lib().ni_relax_colouring
+    fn.restype = ctypes.c_int
+    fn.argtypes = [L._PI64, L._PI32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)]
+    plan = (ctypes.c_int32 * 5)()
+    L.check(fn(L.i64arr(shape), _bounds(bound), _dvec(penalty), plan), "relax_colouring")
+    if plan[0]:
+        return "linear", plan[1], (plan[2], plan[3], plan[4])
+    return "axes", plan[1], None
+
+
 def relax_grid_cf(g, x, hessian=None, penalty=(0, 0, 0, 0, 0), voxel_size=(1, 1, 1),
                   bound="dft", nb_iter=2, weight=None):
     """nb_iter Gauss-Seidel sweeps, x updated in place."""

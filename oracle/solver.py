@@ -37,6 +37,8 @@ def _setup():
     L.or_pcg_grid.restype = I
     L.or_precond_grid.argtypes = [D, D, D, I, I64, I64, I64, I64, PI, D] + [_c.c_double] * 5 + [D]
     L.or_precond_grid.restype = None
+    L.or_relax_plan.argtypes = [I64, I64, I64, PI, I, PI]
+    L.or_relax_plan.restype = I
     L._solver_ready = True
     return L
 
@@ -90,19 +92,36 @@ def regulariser_grid(x, hessian=None, absolute=0, membrane=0, bending=0, lame=0,
 
 
 def relax_grid(g, hessian=None, x=None, absolute=0, membrane=0, bending=0, lame=0,
-               voxel_size=1, bound="dft", nb_iter=1, quirk_fold=False, product_order=False, weight=None):
+               voxel_size=1, bound="dft", nb_iter=1, quirk_fold=False, product_order=False, weight=None,
+               reverse_within_classes=False):
     """Gauss-Seidel sweeps.  quirk_fold: the reference's fold order and sign
-    typos; product_order: the wrap-safe colour classes the HIP kernels use."""
+    typos; product_order: the colour classes the HIP kernels use (see
+    relax_plan); reverse_within_classes (with product_order): visit each
+    class's voxels in reverse order, which changes nothing exactly when
+    every class is an independent set of the stencil."""
     L = _setup()
     g, h = _f64(g), _f64(hessian)
     x = np.zeros_like(g) if x is None else _f64(x).copy()
     N, _, X, Y, Z = g.shape
     CC = 0 if h is None else h.shape[1]
-    mode = 2 if product_order else int(bool(quirk_fold))
+    mode = (3 if reverse_within_classes else 2) if product_order else int(bool(quirk_fold))
     w = _w(weight, g.shape)
     L.or_relax_grid(_d(g), _d(h), CC, _d(x), N, X, Y, Z, _b(bound), _vs(voxel_size),
                     *_pen(absolute, membrane, bending, lame), int(nb_iter), mode, _d(w))
     return x
+
+
+def relax_plan(shape, bound="dft", band=3):
+    """The product's colour classes for a lattice: ("linear", m, (a, b, c))
+    -- colour (a x + b y + c z) mod m, classes 0 .. m-1 in turn -- or
+    ("axes", n_classes, None), the per-axis period classes.  band: 3 with a
+    bending energy, 2 with Lame only, 0 membrane only, 1 absolute only."""
+    L = _setup()
+    abc = (_c.c_int * 3)()
+    m = L.or_relax_plan(*[int(s) for s in shape], _b(bound), int(band), abc)
+    if m:
+        return "linear", m, tuple(abc)
+    return "axes", abc[0], None
 
 
 def prolong(x, out_shape, bound="dct2"):

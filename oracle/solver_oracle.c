@@ -32,7 +32,9 @@
  *                 (relax_grid_common.cpp:469-470), and the sign typos of
  *                 relax3d_all's third component (:1154-1158) are kept;
  *                 0 = the reference's colour order without those quirks;
- *                 2 = the product's wrap-safe colour classes (axis_colours);
+ *                 2 = the product's colour classes (lin_colouring, else
+ *                 axis_colours), 3 = the same visited in reverse within each
+ *                 class (equal to 2 exactly when every class is independent);
  *   quirk_zscale: FMG rescales Z-slices instead of displacement channels
  *                 (fmg_common.cpp:392-489).
  * With both off, the restatement follows the intended semantics (per-channel
@@ -300,6 +302,64 @@ static void relax_voxel(double *x3, const double *g3, const double *h, int CC, c
   x3[i] += d0; x3[V + i] += d1; x3[2 * V + i] += d2;
 }
 
+/* The product's colour classes (mode 2).
+ *
+ * Linear colouring: voxel (x, y, z) has colour (a x + b y + c z) mod m.  A
+ * class is an independent set when no coupling offset v of the stencil
+ * (bending: |v|_1 <= 2; Lame only: |v|_1 <= 2 and |v|_inf <= 1; membrane
+ * only: |v|_1 = 1) has a v0 + b v1 + c v2 = 0 (mod m), where on a circulant
+ * axis v is taken modulo its length (then coefficient * length = 0 (mod m)
+ * is needed for the colour to be defined on the torus); on the other bounds
+ * a folded neighbour is never farther than the unfolded one.  The smallest m
+ * (up to 32) wins, ties by the smallest gcd(c, m), then c, a, b; the class
+ * order is colour 0, 1, ..., m - 1.  Shapes without such a colouring (odd
+ * circulant lengths, for instance) use the per-axis period classes below. */
+typedef struct { int m, a, b, c; } lincol_t;
+
+static int gcd_i(int u, int v) { while (v) { int t = u % v; u = v; v = t; } return u; }
+
+static int lin_colouring(const int64_t sz[3], const int *bound, int bw, lincol_t *out)
+{
+  int off[124][3], no = 0;
+  for (int i = -2; i <= 2; ++i)
+    for (int j = -2; j <= 2; ++j)
+      for (int k = -2; k <= 2; ++k) {
+        int v[3] = {i, j, k}, l1 = abs(i) + abs(j) + abs(k), li = 0;
+        for (int d = 0; d < 3; ++d) if (abs(v[d]) > li) li = abs(v[d]);
+        int coupled = bw == 3 ? l1 <= 2 : (bw == 2 ? (l1 <= 2 && li <= 1) : (bw == 0 ? l1 == 1 : 0));
+        if (!l1 || !coupled) continue;
+        int ok = 1, nz = 0;
+        for (int d = 0; d < 3; ++d) {
+          int64_t n = sz[d];
+          if (n == 1) v[d] = 0;
+          else if (bound[d] == B_DFT) { int64_t r = pmod(v[d], n); if (r > n / 2) r -= n; v[d] = (int)r; }
+          else if (abs(v[d]) > n - 1) ok = 0;
+          nz |= v[d] != 0;
+        }
+        if (ok && nz) { off[no][0] = v[0]; off[no][1] = v[1]; off[no][2] = v[2]; ++no; }
+      }
+  for (int m = 1; m <= 32; ++m) {
+    int found = 0, bg = 0;
+    for (int c = 0; c < m; ++c) {
+      int g = gcd_i(c, m);
+      if (found && g >= bg) continue;
+      for (int a = 0; a < m; ++a)
+        for (int b = 0; b < m; ++b) {
+          int co[3] = {a, b, c}, ok = 1;
+          for (int d = 0; d < 3 && ok; ++d)
+            if (bound[d] == B_DFT && sz[d] > 1 && (co[d] * sz[d]) % m) ok = 0;
+          for (int o = 0; o < no && ok; ++o)
+            if (pmod((int64_t)a * off[o][0] + (int64_t)b * off[o][1] + (int64_t)c * off[o][2], m) == 0) ok = 0;
+          if (ok && (!found || g < bg)) {
+            found = 1; bg = g; out->m = m; out->a = a; out->b = b; out->c = c;
+          }
+        }
+    }
+    if (found) return 1;
+  }
+  return 0;
+}
+
 /* Colour classes along one axis of length n for period F: class c covers
  * start + F*i.  With circulant bounds and n % F != 0 the wrap-around would put
  * two coupled voxels in one class, so the last n % F slices get classes of
@@ -321,7 +381,10 @@ static void axis_colours(colours_t *c, int64_t n, int64_t F, int bound, int safe
 }
 
 /* order: 0 = the reference's folds (quirk_fold reproduces its start indices),
- *        1 = the product's wrap-safe colour classes (see axis_colours). */
+ *        1 = the same with the reference's quirks,
+ *        2 = the product's colour classes (lin_colouring, else axis_colours),
+ *        3 = as 2 with every class visited in reverse voxel order (a class is
+ *            an independent set exactly when this changes nothing). */
 void or_relax_grid(const double *g, const double *h, int CC, double *x,
                    int64_t N, int64_t X, int64_t Y, int64_t Z, const int *bound,
                    const double *voxel_size, double absolute, double membrane,
@@ -331,16 +394,20 @@ void or_relax_grid(const double *g, const double *h, int CC, double *x,
   weights_t W; set_weights(&W, voxel_size, absolute, membrane, bending, lame_shear, lame_div);
   geom_t G = {X, Y, Z, {bound[0], bound[1], bound[2]}};
   int64_t V = X * Y * Z;
-  int safe = quirk_fold == 2;           /* 2: product ordering */
+  int safe = quirk_fold >= 2;           /* 2, 3: product ordering */
+  int rev = quirk_fold == 3;
   int quirk = quirk_fold == 1;
   /* bandwidth: bending 3, Lame 2, membrane 0 (red/black), else 1 */
   int bw = bending ? 3 : ((lame_shear || lame_div) ? 2 : (membrane ? 0 : 1));
-  if (safe && bw == 0) {
-    /* red/black is not an independent set when a circulant axis is odd */
-    for (int d = 0; d < 3; ++d) {
-      int64_t n = d == 0 ? X : (d == 1 ? Y : Z);
-      if (bound[d] == B_DFT && (n & 1) && n > 1) bw = 2;
-    }
+  lincol_t L = {0, 0, 0, 0};
+  int lin = 0, F = bw == 3 ? 3 : (bw == 1 ? 1 : 2);
+  colours_t cx, cy, cz;
+  if (safe) {
+    const int64_t sz[3] = {X, Y, Z};
+    axis_colours(&cx, X, F, bound[0], 1);
+    axis_colours(&cy, Y, F, bound[1], 1);
+    axis_colours(&cz, Z, F, bound[2], 1);
+    lin = lin_colouring(sz, bound, bw, &L) && L.m <= cx.n * cy.n * cz.n;
   }
   for (int64_t n = 0; n < N; ++n) {
     double *xn = x + n * 3 * V;
@@ -348,27 +415,36 @@ void or_relax_grid(const double *g, const double *h, int CC, double *x,
     const double *hn = h ? h + n * CC * V : NULL;
     const double *wn = w ? w + n * V : NULL;
     for (int it = 0; it < nb_iter; ++it) {
+      if (safe && lin) {
+        for (int col = 0; col < L.m; ++col)
+          for (int64_t t = 0; t < V; ++t) {
+            int64_t i = rev ? V - 1 - t : t;
+            int64_t xx = i / (Y * Z), yy = (i / Z) % Y, zz = i % Z;
+            if (pmod(L.a * xx + L.b * yy + L.c * zz, L.m) == col)
+              relax_voxel(xn, gn, hn, CC, &G, &W, V, xx, yy, zz, 0, wn);
+          }
+        continue;
+      }
+      if (safe) {
+        for (int a = 0; a < cx.n; ++a)
+          for (int b = 0; b < cy.n; ++b)
+            for (int c = 0; c < cz.n; ++c) {
+              int64_t cnt = cx.count[a] * cy.count[b] * cz.count[c];
+              for (int64_t t = 0; t < cnt; ++t) {
+                int64_t u = rev ? cnt - 1 - t : t;
+                int64_t i = u / (cy.count[b] * cz.count[c]), j = (u / cz.count[c]) % cy.count[b], k = u % cz.count[c];
+                relax_voxel(xn, gn, hn, CC, &G, &W, V, cx.start[a] + cx.step[a] * i,
+                            cy.start[b] + cy.step[b] * j, cz.start[c] + cz.step[c] * k, 0, wn);
+              }
+            }
+        continue;
+      }
       if (bw == 0) {
         for (int rb = 0; rb < 2; ++rb)
           for (int64_t xx = 0; xx < X; ++xx)
             for (int64_t yy = 0; yy < Y; ++yy)
               for (int64_t zz = 0; zz < Z; ++zz)
                 if ((xx + yy + zz) % 2 == rb) relax_voxel(xn, gn, hn, CC, &G, &W, V, xx, yy, zz, quirk, wn);
-        continue;
-      }
-      if (safe) {
-        colours_t cx, cy, cz;
-        axis_colours(&cx, X, bw, bound[0], 1);
-        axis_colours(&cy, Y, bw, bound[1], 1);
-        axis_colours(&cz, Z, bw, bound[2], 1);
-        for (int a = 0; a < cx.n; ++a)
-          for (int b = 0; b < cy.n; ++b)
-            for (int c = 0; c < cz.n; ++c)
-              for (int64_t i = 0; i < cx.count[a]; ++i)
-                for (int64_t j = 0; j < cy.count[b]; ++j)
-                  for (int64_t k = 0; k < cz.count[c]; ++k)
-                    relax_voxel(xn, gn, hn, CC, &G, &W, V, cx.start[a] + cx.step[a] * i,
-                                cy.start[b] + cy.step[b] * j, cz.start[c] + cz.step[c] * k, 0, wn);
         continue;
       }
       int64_t Fx = X < bw ? X : bw, Fy = Y < bw ? Y : bw, Fz = Z < bw ? Z : bw;
@@ -383,6 +459,26 @@ void or_relax_grid(const double *g, const double *h, int CC, double *x,
       }
     }
   }
+}
+
+/* Colouring the product uses for a shape, bounds and bandwidth: returns m
+ * (linear colouring, coefficients in abc) or 0 (per-axis period classes,
+ * their class count in abc[0]). */
+int or_relax_plan(int64_t X, int64_t Y, int64_t Z, const int *bound, int bw, int *abc)
+{
+  const int64_t sz[3] = {X, Y, Z};
+  int F = bw == 3 ? 3 : (bw == 1 ? 1 : 2);
+  colours_t cx, cy, cz;
+  axis_colours(&cx, X, F, bound[0], 1);
+  axis_colours(&cy, Y, F, bound[1], 1);
+  axis_colours(&cz, Z, F, bound[2], 1);
+  lincol_t L;
+  if (lin_colouring(sz, bound, bw, &L) && L.m <= cx.n * cy.n * cz.n) {
+    abc[0] = L.a; abc[1] = L.b; abc[2] = L.c;
+    return L.m;
+  }
+  abc[0] = cx.n * cy.n * cz.n; abc[1] = abc[2] = 0;
+  return 0;
 }
 
 /* ---------------------------------------------------------------------- */

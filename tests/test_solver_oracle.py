@@ -260,3 +260,90 @@ def test_relax_and_fmg_converge_to_cg():
         x = S.fmg_grid(h, g, nb_cycles=cycles, nb_iter=4, **kw)
         res.append(np.linalg.norm(g - S.regulariser_grid(x, h, **kw)) / np.linalg.norm(g))
     assert res[0] > res[1] > res[2] and res[2] < 1e-4
+
+
+BAND_PEN = {3: dict(absolute=0.01, membrane=0.1, bending=0.3, lame=(0.05, 0.2)),
+            2: dict(absolute=0.01, membrane=0.1, lame=(0.1, 0.3)),
+            0: dict(absolute=0.01, membrane=0.2)}
+
+
+def _coupling_pairs(shape, bound, pen):
+    """Voxel pairs (i, j), i != j, that the operator (H + L) couples: the
+    regulariser applied to every unit impulse (one per voxel and component,
+    as a batch), with random voxel sizes so that no weight vanishes."""
+    X, Y, Z = shape
+    V = X * Y * Z
+    e = np.zeros((3 * V, 3, V))
+    for d in range(3):
+        e[np.arange(V) + d * V, d, np.arange(V)] = 1.0
+    e = e.reshape(3 * V, 3, X, Y, Z)
+    out = S.regulariser_grid(e, None, voxel_size=(0.9, 1.1, 1.3), bound=bound, **pen)
+    reach = np.abs(out).reshape(3, V, 3, V).max((0, 2)) > 0      # [source voxel, target voxel]
+    np.fill_diagonal(reach, False)
+    return np.argwhere(reach)
+
+
+def _linear_colours(shape, m, abc):
+    x, y, z = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+    return ((abc[0] * x + abc[1] * y + abc[2] * z) % m).ravel()
+
+
+@pytest.mark.parametrize("bound", ["dft", "dct2", "dst1", "replicate"])
+@pytest.mark.parametrize("band", [3, 2, 0])
+def test_relax_colour_classes_are_independent(bound, band):
+    """The product's colour classes never hold two voxels that the stencil
+    couples (checked on the operator's actual coupling graph), so a sweep
+    over a class is order-free: visiting each class in reverse gives the
+    same iterate, bit for bit.  Both linear colourings and per-axis classes
+    (shapes without a linear colouring) are covered."""
+    rng = np.random.default_rng(band)
+    pen = BAND_PEN[band]
+    kinds = set()
+    shapes = [(8, 8, 8), (8, 6, 10), (5, 6, 4), (7, 5, 3)]
+    if bound == "dft":
+        shapes.append((10, 12, 10))     # bending: 15 classes, z coefficient 3
+    for shape in shapes:
+        kind, m, abc = S.relax_plan(shape, bound, band)
+        kinds.add(kind)
+        if kind == "linear" and (bound == "dft" or shape[0] * shape[1] * shape[2] <= 200):
+            col = _linear_colours(shape, m, abc)
+            pairs = _coupling_pairs(shape, bound, pen)
+            assert len(pairs) > 0
+            assert not (col[pairs[:, 0]] == col[pairs[:, 1]]).any(), (shape, m, abc)
+        g = rng.standard_normal((2, 3) + shape)
+        h = np.concatenate([rng.uniform(0.5, 2, (2, 3) + shape), rng.uniform(-0.2, 0.2, (2, 3) + shape)], 1)
+        x0 = rng.standard_normal((2, 3) + shape)
+        a = S.relax_grid(g, h, x0, bound=bound, nb_iter=2, product_order=True, **pen)
+        b = S.relax_grid(g, h, x0, bound=bound, nb_iter=2, product_order=True, reverse_within_classes=True, **pen)
+        assert np.array_equal(a, b), (shape, kind, m)
+        assert not np.array_equal(a, x0)
+    if bound == "dft" and band != 0:
+        assert kinds == {"linear", "axes"}
+
+
+def test_relax_colouring_check_detects_conflicts():
+    """Planted mistake: the 8-colouring (2x + 3y + z) mod 8 of a 16^3 dft
+    lattice used on a 10^3 dft lattice (8 does not divide 10) puts coupled
+    voxels across the wrap into one class; the coupling-graph check sees it,
+    and so does a sweep in reverse class order."""
+    shape = (10, 10, 10)
+    assert S.relax_plan((16, 16, 16), "dft", 3)[:2] == ("linear", 8)
+    col = _linear_colours(shape, 8, (2, 3, 1))
+    pairs = _coupling_pairs(shape, "dft", BAND_PEN[3])
+    assert (col[pairs[:, 0]] == col[pairs[:, 1]]).any()
+    # and the product plan for 10^3 differs from that colouring
+    assert S.relax_plan(shape, "dft", 3) != ("linear", 8, (2, 3, 1))
+
+
+def test_relax_plan_sizes():
+    """Class counts of the product's plans on the fmg_160 pyramid (dft,
+    bending): 8 classes on 160x192x160 .. 40x48x40, 15 below; 7 with
+    reflecting bounds; red/black for membrane only."""
+    for shape in [(160, 192, 160), (80, 96, 80), (40, 48, 40)]:
+        assert S.relax_plan(shape, "dft", 3) == ("linear", 8, (2, 3, 1))
+        assert S.relax_plan(shape, "dct2", 3) == ("linear", 7, (2, 3, 1))
+        assert S.relax_plan(shape, "dft", 0) == ("linear", 2, (1, 1, 1))
+    for shape in [(20, 24, 20), (10, 12, 10)]:
+        assert S.relax_plan(shape, "dft", 3)[:2] == ("linear", 15)
+    # no linear colouring of an odd circulant lattice with <= 60 classes
+    assert S.relax_plan((7, 5, 9), "dft", 3)[0] == "axes"
