@@ -372,18 +372,12 @@ __global__ __launch_bounds__(SV_BLOCK) void reg_kernel(const T *__restrict__ x, 
   }
 }
 
-// One Gauss-Seidel update of voxel (xx, y, z) of batch n.
+// Gauss-Seidel update of one voxel: x += (H (1 + 1e-6) + diag)^-1
+// (g - (L x) - H x), given (L x), the diagonal, g and H there.
 template <typename T>
-__device__ __forceinline__ void relax_voxel(T *__restrict__ xn, const T *__restrict__ gn, const T *__restrict__ hn,
-                                            int CC, int V, const Geo &G, const StencilT<T> &S, int n,
-                                            int xx, int y, int z) {
-  const int i = (xx * G.Y + y) * G.Z + z;
-  T Lx[3], dg[3];
-  apply_op<T>(xn, n, V, G, S, xx, y, z, Lx, dg);
-  T r0 = gn[i] - Lx[0], r1 = gn[V + i] - Lx[1], r2 = gn[2 * V + i] - Lx[2];
-  T M[6];
-  load_H<T>(hn, hn ? CC : 0, V, i, M);
-  const T x0 = xn[i], x1 = xn[V + i], x2 = xn[2 * V + i];
+__device__ __forceinline__ void gs_update(const T Lx[3], const T dg[3], T g0, T g1, T g2, const T M[6],
+                                          T &x0, T &x1, T &x2) {
+  T r0 = g0 - Lx[0], r1 = g1 - Lx[1], r2 = g2 - Lx[2];
   r0 -= M[0] * x0 + M[3] * x1 + M[4] * x2;
   r1 -= M[3] * x0 + M[1] * x1 + M[5] * x2;
   r2 -= M[4] * x0 + M[5] * x1 + M[2] * x2;
@@ -392,9 +386,27 @@ __device__ __forceinline__ void relax_voxel(T *__restrict__ xn, const T *__restr
   const T p = M[3], q = M[4], s = M[5];
   // 3x3 symmetric solve by the adjugate (relax_grid_common.cpp:840-852)
   const T idt = T(1) / (a * b * c - a * s * s - b * q * q - c * p * p + T(2) * p * q * s);
-  xn[i] = x0 + idt * (r0 * (b * c - s * s) + r1 * (q * s - p * c) + r2 * (p * s - q * b));
-  xn[V + i] = x1 + idt * (r0 * (q * s - p * c) + r1 * (a * c - q * q) + r2 * (p * q - a * s));
-  xn[2 * V + i] = x2 + idt * (r0 * (p * s - q * b) + r1 * (p * q - a * s) + r2 * (a * b - p * p));
+  const T n0 = x0 + idt * (r0 * (b * c - s * s) + r1 * (q * s - p * c) + r2 * (p * s - q * b));
+  const T n1 = x1 + idt * (r0 * (q * s - p * c) + r1 * (a * c - q * q) + r2 * (p * q - a * s));
+  const T n2 = x2 + idt * (r0 * (p * s - q * b) + r1 * (p * q - a * s) + r2 * (a * b - p * p));
+  x0 = n0; x1 = n1; x2 = n2;
+}
+
+// One Gauss-Seidel update of voxel (xx, y, z) of batch n.
+template <typename T>
+__device__ __forceinline__ void relax_voxel(T *__restrict__ xn, const T *__restrict__ gn, const T *__restrict__ hn,
+                                            int CC, int V, const Geo &G, const StencilT<T> &S, int n,
+                                            int xx, int y, int z) {
+  const int i = (xx * G.Y + y) * G.Z + z;
+  T Lx[3], dg[3];
+  apply_op<T>(xn, n, V, G, S, xx, y, z, Lx, dg);
+  T M[6];
+  load_H<T>(hn, hn ? CC : 0, V, i, M);
+  T x0 = xn[i], x1 = xn[V + i], x2 = xn[2 * V + i];
+  gs_update<T>(Lx, dg, gn[i], gn[V + i], gn[2 * V + i], M, x0, x1, x2);
+  xn[i] = x0;
+  xn[V + i] = x1;
+  xn[2 * V + i] = x2;
 }
 
 // Colour classes of a multi-colour sweep.  Each class is an independent set
@@ -425,6 +437,7 @@ struct AxisColours {
 struct LinColour {
   int m, a, b, c;
   int gz, s, cinv;         // gcd(c, m), m / gz, inverse of c / gz modulo s
+  unsigned magic;          // ceil(2^32 / m): v / m = umulhi(v, magic) for v < 2^27
 };
 
 struct ColourPlan {
@@ -480,57 +493,218 @@ __global__ __launch_bounds__(SV_BLOCK) void relax_colour(T *__restrict__ x, cons
                  CC, V, G, S, n, xx, y, z);
 }
 
-// Small levels: one work-group runs every sweep and colour class with
-// barriers in between (threads enumerate the members of each class), on a copy of the solution held in LDS (3 N V values, up to the
-// device's LDS per work-group), so the 75 stencil reads per voxel and the
-// updates never leave the CU.  32-bit index arithmetic (these levels are small).
-// Double precision runs 512 threads, so that its stencil state fits the
-// registers of a 512-thread work-group without spilling to scratch.
-template <typename T>
-constexpr int relax_small_threads() { return sizeof(T) == 8 ? 512 : 1024; }
+// Class-major layout of a field with C channels under a linear colouring
+// with gcd(c, m) = 1 (then s = m): value (n, ch, x, y, z) of colour
+// col = (a x + b y + c z) mod m sits at slot (x Y + y) K + z / m of block
+// (col N + n) C + ch, K = ceil(Z / m) slots per row.  A class's values are
+// contiguous, consecutive members of a row are consecutive slots, and a
+// stencil neighbour of consecutive members is at consecutive slots of its own
+// class: every load of a class sweep is coalesced and reads only what it
+// uses, where the natural layout makes each launch fetch every line of g, H
+// and x (the members sit m voxels apart).
+__device__ __forceinline__ unsigned cm_div(unsigned v, const LinColour &L) {
+  return __umulhi(v, L.magic);
+}
 
+template <typename T, bool TO_CM>
+__global__ __launch_bounds__(SV_BLOCK) void cm_permute(const T *__restrict__ src, T *__restrict__ dst, int C, int N,
+                                                       Geo G, LinColour L) {
+  const long long V = (long long)G.X * G.Y * G.Z;
+  const long long gid = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
+  if (gid >= V * C * N) return;
+  const int nc = (int)(gid / V);
+  const int i = (int)(gid - (long long)nc * V);
+  const int z = i % G.Z, row = i / G.Z, y = row % G.Y, x = row / G.Y;
+  const unsigned v = (unsigned)(L.a * x + L.b * y + L.c * z);
+  const int col = (int)(v - cm_div(v, L) * L.m);
+  const int K = (G.Z + L.m - 1) / L.m;
+  const long long XYK = (long long)G.X * G.Y * K;
+  const long long j = ((long long)col * N * C + nc) * XYK + (long long)row * K + (int)cm_div((unsigned)z, L);
+  if (TO_CM) dst[j] = src[gid];
+  else dst[gid] = src[j];
+}
+
+// One linear class of a sweep on class-major x, g, H (no RLS weights).
 template <typename T>
+__global__ __launch_bounds__(SV_BLOCK) void relax_cm(T *__restrict__ xp, const T *__restrict__ gp,
+                                                     const T *__restrict__ hp, int CC, Geo G, StencilT<T> S, int N,
+                                                     LinColour L, int col) {
+  const int K = (G.Z + L.m - 1) / L.m;
+  const int XYK = G.X * G.Y * K;
+  const long long gid = sv_block() * SV_BLOCK + threadIdx.x;
+  if (gid >= (long long)XYK * N) return;
+  const int n = (int)(gid / XYK), slot = (int)(gid - (long long)n * XYK);
+  const int row = slot / K, k = slot - row * K, y = row % G.Y, x = row / G.Y;
+  const int z = lin_member_z(L, col, x, y, k, G.Z);
+  if (z < 0) return;
+  // folded taps -2..2 per axis: row / slot part of the address, colour
+  // contribution (coefficient * folded coordinate mod m) and sign
+  int xr[5], yr[5], zk[5], ca[5], cb[5], cz[5], sx[5], sy[5], sz[5];
+#pragma unroll
+  for (int t = 0; t < 5; ++t) {
+    const int xf = fold<int>(G.b[0], x + t - 2, G.X, sx[t]);
+    const int yf = fold<int>(G.b[1], y + t - 2, G.Y, sy[t]);
+    const int zf = fold<int>(G.b[2], z + t - 2, G.Z, sz[t]);
+    xr[t] = xf * G.Y;
+    yr[t] = yf;
+    zk[t] = (int)cm_div((unsigned)zf, L);
+    unsigned v = (unsigned)(L.a * xf);
+    ca[t] = (int)(v - cm_div(v, L) * L.m);
+    v = (unsigned)(L.b * yf);
+    cb[t] = (int)(v - cm_div(v, L) * L.m);
+    v = (unsigned)(L.c * zf);
+    cz[t] = (int)(v - cm_div(v, L) * L.m);
+  }
+  const T *xn = xp + (size_t)n * 3 * XYK;
+  const size_t cstride = (size_t)N * 3 * XYK;     // one class block
+  auto at = [&](int c, int i, int j, int kk) -> T {
+    int cls = ca[i + 2] + cb[j + 2] + cz[kk + 2];
+    cls = cls >= L.m ? cls - L.m : cls;
+    cls = cls >= L.m ? cls - L.m : cls;
+    const T v = xn[cls * cstride + (size_t)c * XYK + (size_t)(xr[i + 2] + yr[j + 2]) * K + zk[kk + 2]];
+    if (i == 0 && j == 0 && kk == 0) return v;
+    const int sg = sx[i + 2] * sy[j + 2] * sz[kk + 2];
+    return sg == 0 ? T(0) : (sg < 0 ? -v : v);
+  };
+  T Lx[3];
+  apply_L_core<T>(at, S, Lx);
+  const size_t own = (size_t)col * cstride + (size_t)n * 3 * XYK + slot;
+  T M[6];
+  load_H<T>(hp ? hp + ((size_t)col * N + n) * CC * XYK : nullptr, hp ? CC : 0, XYK, slot, M);
+  T x0 = xp[own], x1 = xp[own + XYK], x2 = xp[own + 2 * (size_t)XYK];
+  gs_update<T>(Lx, S.diag, gp[own], gp[own + XYK], gp[own + 2 * (size_t)XYK], M, x0, x1, x2);
+  xp[own] = x0;
+  xp[own + XYK] = x1;
+  xp[own + 2 * (size_t)XYK] = x2;
+}
+
+// Small levels: one work-group runs every sweep and colour class with
+// barriers in between, on a copy of the solution held in LDS (3 N V values,
+// up to the device's LDS per work-group), so the 75 stencil reads per voxel
+// and the updates never leave the CU.  At the start it sorts the voxels by
+// class into LDS member lists (uint16 voxel indices: such levels have far
+// fewer than 65536 voxels; the order inside a class is free, a class being an
+// independent set) and tabulates each axis's folded taps, so that a class
+// step hands one member to each thread with no index or fold arithmetic.
+// RLS: voxel-wise weights (the weighted operator, read through apply_op).
+// Double precision runs 256 threads, so that its stencil state fits the
+// registers without spilling to scratch.
+template <typename T>
+constexpr int relax_small_threads() { return sizeof(T) == 8 ? 256 : 1024; }
+
+__host__ __device__ inline int plan_classes(const ColourPlan &P) {
+  return P.lin ? P.L.m : P.ax[0].n * P.ax[1].n * P.ax[2].n;
+}
+
+// LDS bytes relax_small needs: x, fold tables, member lists, class counters.
+template <typename T>
+static size_t small_lds_need(const Geo &G, int N, const ColourPlan &P) {
+  const size_t V = (size_t)G.X * G.Y * G.Z;
+  size_t b = 3 * (size_t)N * V * sizeof(T) + 5 * (size_t)(G.X + G.Y + G.Z) * sizeof(int);
+  b += (V * sizeof(unsigned short) + 3) & ~(size_t)3;
+  return b + 2 * (size_t)(plan_classes(P) + 1) * sizeof(int);
+}
+
+// Class of voxel (x, y, z) under plan P (per-axis plans: the class triple in
+// sweep order, x outermost).
+__device__ __forceinline__ int plan_class(const ColourPlan &P, int xx, int y, int z) {
+  if (P.lin) return (P.L.a * xx + P.L.b * y + P.L.c * z) % P.L.m;
+  const int co[3] = {xx, y, z};
+  int k[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    k[d] = 0;
+    for (int q = 0; q < P.ax[d].n; ++q) {
+      const int r = co[d] - P.ax[d].start[q];
+      if (r >= 0 && r % P.ax[d].step[q] == 0 && r / P.ax[d].step[q] < P.ax[d].count[q]) { k[d] = q; break; }
+    }
+  }
+  return (k[0] * P.ax[1].n + k[1]) * P.ax[2].n + k[2];
+}
+
+template <typename T, bool RLS>
 __global__ __launch_bounds__(relax_small_threads<T>()) void relax_small(T *__restrict__ x, const T *__restrict__ g, const T *__restrict__ h,
                                                     int CC, Geo G, StencilT<T> S, int N, ColourPlan P, int nb_iter) {
   extern __shared__ char sv_smem[];
+  const int V = G.X * G.Y * G.Z, YZ = G.Y * G.Z;
+  const int ncls = plan_classes(P);
   T *xs = (T *)sv_smem;
-  const int V = G.X * G.Y * G.Z;
+  int *ft = (int *)(sv_smem + 3 * (size_t)N * V * sizeof(T));      // [X + Y + Z][5]
+  unsigned short *mem = (unsigned short *)(ft + 5 * (G.X + G.Y + G.Z));
+  int *cnt = (int *)((char *)mem + ((V * sizeof(unsigned short) + 3) & ~(size_t)3));
+  int *start = cnt + ncls + 1;
   for (int i = threadIdx.x; i < 3 * N * V; i += blockDim.x) xs[i] = x[i];
-  __syncthreads();
-  for (int it = 0; it < nb_iter; ++it) {
-    if (P.lin) {
-      const int K = (G.Z + P.L.s - 1) / P.L.s;
-      const int per = G.X * G.Y * K;
-      for (int col = 0; col < P.L.m; ++col) {
-        for (int k = threadIdx.x; k < per * N; k += blockDim.x) {
-          const int n = k / per, j = k - n * per;
-          const int row = j / K, y = row % G.Y, xx = row / G.Y;
-          const int z = lin_member_z(P.L, col, xx, y, j - row * K, G.Z);
-          if (z >= 0)
-            relax_voxel<T>(xs + (size_t)n * 3 * V, g + (size_t)n * 3 * V, h ? h + (size_t)n * CC * V : nullptr,
-                           CC, V, G, S, n, xx, y, z);
-        }
-        __syncthreads();
-      }
-      continue;
-    }
-    for (int a = 0; a < P.ax[0].n; ++a)
-      for (int b = 0; b < P.ax[1].n; ++b)
-        for (int c = 0; c < P.ax[2].n; ++c) {
-          const int nx = P.ax[0].count[a], ny = P.ax[1].count[b], nz = P.ax[2].count[c];
-          const int nyz = ny * nz, per = nx * nyz;
-          for (int k = threadIdx.x; k < per * N; k += blockDim.x) {
-            const int n = k / per, j = k - n * per;
-            const int jx = j / nyz, r = j - jx * nyz, jy = r / nz, jz = r - jy * nz;
-            const int z = P.ax[2].start[c] + P.ax[2].step[c] * jz;
-            const int y = P.ax[1].start[b] + P.ax[1].step[b] * jy;
-            const int xx = P.ax[0].start[a] + P.ax[0].step[a] * jx;
-            relax_voxel<T>(xs + (size_t)n * 3 * V, g + (size_t)n * 3 * V, h ? h + (size_t)n * CC * V : nullptr,
-                           CC, V, G, S, n, xx, y, z);
-          }
-          __syncthreads();
-        }
+  for (int i = threadIdx.x; i < 5 * (G.X + G.Y + G.Z); i += blockDim.x) {
+    const int c = i / 5, t = i - 5 * c;
+    const int d = c < G.X ? 0 : (c < G.X + G.Y ? 1 : 2);
+    const int co = c - (d == 0 ? 0 : (d == 1 ? G.X : G.X + G.Y));
+    const int n = d == 0 ? G.X : (d == 1 ? G.Y : G.Z);
+    int sg;
+    const int f = fold<int>(G.b[d], co + t - 2, n, sg);
+    ft[i] = (f << 2) | (sg + 1);
   }
+  for (int i = threadIdx.x; i <= ncls; i += blockDim.x) cnt[i] = 0;
+  __syncthreads();
+  for (int v = threadIdx.x; v < V; v += blockDim.x)
+    atomicAdd(&cnt[plan_class(P, v / YZ, (v / G.Z) % G.Y, v % G.Z)], 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int k = 0; k < ncls; ++k) { start[k] = acc; acc += cnt[k]; cnt[k] = 0; }
+    start[ncls] = acc;
+  }
+  __syncthreads();
+  for (int v = threadIdx.x; v < V; v += blockDim.x) {
+    const int k = plan_class(P, v / YZ, (v / G.Z) % G.Y, v % G.Z);
+    mem[start[k] + atomicAdd(&cnt[k], 1)] = (unsigned short)v;
+  }
+  __syncthreads();
+  const int *ftx = ft, *fty = ft + 5 * G.X, *ftz = ft + 5 * (G.X + G.Y);
+  // folds that flip signs or zero values: dst1, dst2, zero
+  bool signs = false;
+  for (int d = 0; d < 3; ++d) signs = signs || G.b[d] == B_DST1 || G.b[d] == B_DST2 || G.b[d] == B_ZERO;
+  for (int it = 0; it < nb_iter; ++it)
+    for (int col = 0; col < ncls; ++col) {
+      const int c0 = start[col], cn = start[col + 1] - c0;
+      for (int k = threadIdx.x; k < cn * N; k += blockDim.x) {
+        const int n = k / cn, v = mem[c0 + k - n * cn];
+        const int xx = v / YZ, y = (v / G.Z) % G.Y, z = v % G.Z;
+        T *xn = xs + (size_t)n * 3 * V;
+        const T *gn = g + (size_t)n * 3 * V;
+        const T *hn = h ? h + (size_t)n * CC * V : nullptr;
+        if (RLS) {
+          relax_voxel<T>(xn, gn, hn, CC, V, G, S, n, xx, y, z);
+          continue;
+        }
+        // per axis and tap: (offset << 2) | (sign + 1)
+        int ox[5], oy[5], oz[5];
+#pragma unroll
+        for (int t = 0; t < 5; ++t) {
+          const int fx = ftx[5 * xx + t], fy = fty[5 * y + t], fz = ftz[5 * z + t];
+          ox[t] = (((fx >> 2) * YZ) << 2) | (fx & 3);
+          oy[t] = (((fy >> 2) * G.Z) << 2) | (fy & 3);
+          oz[t] = fz;
+        }
+        auto at = [&](int c, int i, int j, int kk) -> T {
+          const int a = ox[i + 2], b = oy[j + 2], e = oz[kk + 2];
+          const T val = xn[c * V + (a >> 2) + (b >> 2) + (e >> 2)];
+          if (!signs || (i == 0 && j == 0 && kk == 0)) return val;
+          const int sg = ((a & 3) - 1) * ((b & 3) - 1) * ((e & 3) - 1);
+          return sg == 0 ? T(0) : (sg < 0 ? -val : val);
+        };
+        T Lx[3];
+        apply_L_core<T>(at, S, Lx);
+        const int i = v;
+        T M[6];
+        load_H<T>(hn, hn ? CC : 0, V, i, M);
+        T x0 = xn[i], x1 = xn[V + i], x2 = xn[2 * V + i];
+        gs_update<T>(Lx, S.diag, gn[i], gn[V + i], gn[2 * V + i], M, x0, x1, x2);
+        xn[i] = x0;
+        xn[V + i] = x1;
+        xn[2 * V + i] = x2;
+      }
+      __syncthreads();
+    }
   for (int i = threadIdx.x; i < 3 * N * V; i += blockDim.x) x[i] = xs[i];
 }
 
@@ -1055,6 +1229,7 @@ static bool lin_colouring(const Geo &G, int band, LinColour &L) {
       const int cr = (L.c / L.gz) % L.s;
       for (int t = 0; t < L.s; ++t)
         if ((cr * t) % L.s == 1 % L.s) { L.cinv = t; break; }
+      L.magic = (unsigned)(((1ull << 32) + L.m - 1) / L.m);
       return true;
     }
   }
@@ -1087,7 +1262,9 @@ static size_t small_lds_bytes() {
   size_t lim = SV_SMALL_BYTES;
   if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
       (size_t)v > lim &&
-      hipFuncSetAttribute((const void *)relax_small<T>, hipFuncAttributeMaxDynamicSharedMemorySize, v) ==
+      hipFuncSetAttribute((const void *)relax_small<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, v) ==
+          hipSuccess &&
+      hipFuncSetAttribute((const void *)relax_small<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, v) ==
           hipSuccess)
     lim = (size_t)v;
   (void)hipGetLastError();
@@ -1099,9 +1276,14 @@ template <typename T>
 static void launch_relax(T *x, const T *g, const T *h, int CC, const Geo &G, const StencilT<T> &S,
                          int N, const ColourPlan &P, int nb_iter, hipStream_t st) {
   const long long V = (long long)G.X * G.Y * G.Z;
-  const size_t xbytes = (size_t)3 * N * V * sizeof(T);
-  if (xbytes <= small_lds_bytes<T>()) {
-    hipLaunchKernelGGL(relax_small<T>, dim3(1), dim3(relax_small_threads<T>()), xbytes, st, x, g, h, CC, G, S, N, P, nb_iter);
+  const size_t lds = small_lds_need<T>(G, N, P);
+  if (V <= 65536 && lds <= small_lds_bytes<T>()) {
+    if (S.wmap)
+      hipLaunchKernelGGL((relax_small<T, true>), dim3(1), dim3(relax_small_threads<T>()), lds, st, x, g, h, CC, G, S, N,
+                         P, nb_iter);
+    else
+      hipLaunchKernelGGL((relax_small<T, false>), dim3(1), dim3(relax_small_threads<T>()), lds, st, x, g, h, CC, G, S, N,
+                         P, nb_iter);
     return;
   }
   for (int it = 0; it < nb_iter; ++it) {
@@ -1249,10 +1431,29 @@ static size_t fmg_ws_elems(int N, int CC, int has_w, const Level *L, int NL) {
 static size_t fmg_norms_offset(int N, int CC, int has_w, const Level *L, int NL, size_t esz) {
   return align_up(fmg_ws_elems(N, CC, has_w, L, NL) * esz, 256);
 }
+// Levels of at least CM_MIN_VOX voxels (N V) relax in the class-major layout
+// when their colouring allows (Gauss-Seidel smoother, no RLS weights): x and
+// g are permuted in and x out around each relax (buffers shared by the
+// levels), H once per solve (a buffer per level).  Sized for any linear
+// colouring with gcd(c, m) = 1, m <= 32: m ceil(Z / m) <= Z + 31 slots a row.
+#define CM_MIN_VOX (1 << 18)
+static bool cm_level(const Level &L, int N) { return (long long)N * (long long)L.V >= CM_MIN_VOX; }
+static size_t cm_slots(const Level &L) { return (size_t)L.sz[0] * L.sz[1] * (L.sz[2] + 31); }
+static size_t cm_ws_elems(int N, int CC, const Level *L, int NL) {
+  size_t mx = 0, hs = 0;
+  for (int l = 0; l < NL; ++l)
+    if (cm_level(L[l], N)) {
+      mx = std::max(mx, cm_slots(L[l]));
+      hs += (size_t)N * CC * cm_slots(L[l]);
+    }
+  return 2 * (size_t)N * 3 * mx + hs;
+}
 static size_t fmg_ws_bytes(int N, int CC, int has_w, const Level *L, int NL, int use_cg, size_t esz) {
   size_t b = fmg_norms_offset(N, CC, has_w, L, NL, esz) + 2 * SV_SLOTS * sizeof(double);
   if (use_cg)
     b = align_up(b + 2 * pcg_field_elems(N, L[0].V) * esz, 256) + 4 * (size_t)N * SV_SLOTS * sizeof(double);
+  else if (!has_w)
+    b = align_up(b, 256) + cm_ws_elems(N, CC, L, NL) * esz;
   return b;
 }
 
@@ -1267,6 +1468,9 @@ struct FMG {
   StencilT<T> S[16];
   Geo G[16];
   ColourPlan P[16];        // Gauss-Seidel colour classes per level
+  int cm[16];              // level relaxes in the class-major layout
+  T *cm_x, *cm_g;          // class-major x and g (shared by the levels)
+  T *cm_h[16];             // class-major H per class-major level
   int bound[3];
   T *cg_z, *cg_p;          // PCG smoother scratch (sized for level 0)
   double *cg_sc;
@@ -1281,7 +1485,18 @@ struct FMG {
     const T *gl = l == 0 ? g0 : g[l];
     if (use_cg)
       run_pcg<T>(x[l], gl, h[l], CC, G[l], S[l], N, nb_iter, 0.0, r[l], cg_z, cg_p, cg_sc, st, nullptr);
-    else
+    else if (cm[l]) {
+      const LinColour &Lc = P[l].L;
+      const long long nv = (long long)N * 3 * L[l].V;
+      const long long slots = (long long)N * G[l].X * G[l].Y * ((G[l].Z + Lc.m - 1) / Lc.m);
+      hipLaunchKernelGGL((cm_permute<T, true>), dim3(nblocks(nv)), dim3(SV_BLOCK), 0, st, x[l], cm_x, 3, N, G[l], Lc);
+      hipLaunchKernelGGL((cm_permute<T, true>), dim3(nblocks(nv)), dim3(SV_BLOCK), 0, st, gl, cm_g, 3, N, G[l], Lc);
+      for (int it = 0; it < nb_iter; ++it)
+        for (int col = 0; col < Lc.m; ++col)
+          hipLaunchKernelGGL(relax_cm<T>, dim3(nblocks(slots)), dim3(SV_BLOCK), 0, st, cm_x, cm_g,
+                             (const T *)cm_h[l], CC, G[l], S[l], N, Lc, col);
+      hipLaunchKernelGGL((cm_permute<T, false>), dim3(nblocks(nv)), dim3(SV_BLOCK), 0, st, cm_x, x[l], 3, N, G[l], Lc);
+    } else
       launch_relax<T>(x[l], gl, h[l], CC, G[l], S[l], N, P[l], nb_iter, st);
   }
   void restrict_field(const T *fine, T *coarse, int l) {   // level l -> l+1, displacement scaling
@@ -1367,6 +1582,23 @@ static int fmg_setup(FMG<T> &F, int N, const int *sz, const T *hess, int CC, con
     for (int d = 0; d < 3; ++d) F.G[l].b[d] = bound[d];
     if (!use_cg) F.P[l] = make_plan(F.G[l], F.band);
   }
+  // class-major buffers (after the norm slots; sized by fmg_ws_bytes)
+  for (int l = 0; l < 16; ++l) { F.cm[l] = 0; F.cm_h[l] = nullptr; }
+  F.cm_x = F.cm_g = nullptr;
+  if (!use_cg && !has_w) {
+    T *q = (T *)((char *)ws + align_up(noff + 2 * SV_SLOTS * sizeof(double), 256));
+    size_t mx = 0;
+    for (int l = 0; l < F.NL; ++l)
+      if (cm_level(F.L[l], N)) mx = std::max(mx, cm_slots(F.L[l]));
+    F.cm_x = q; q += (size_t)N * 3 * mx;
+    F.cm_g = q; q += (size_t)N * 3 * mx;
+    for (int l = 0; l < F.NL; ++l)
+      if (cm_level(F.L[l], N)) {
+        if (F.CC) { F.cm_h[l] = q; q += (size_t)N * F.CC * cm_slots(F.L[l]); }
+        const LinColour &Lc = F.P[l].L;
+        F.cm[l] = F.P[l].lin && Lc.gz == 1 && Lc.m >= 2;
+      }
+  }
   // pyramid of Hessians, gradients and initial solutions
   for (int l = 1; l < F.NL; ++l) {
     if (F.CC) F.restrict_hess(F.h[l - 1], (T *)F.h[l], l - 1);
@@ -1375,6 +1607,12 @@ static int fmg_setup(FMG<T> &F, int N, const int *sz, const T *hess, int CC, con
     F.restrict_field(l == 1 ? grad : F.g[l - 1], F.g[l], l - 1);
     F.restrict_field(F.x[l - 1], F.x[l], l - 1);
   }
+  for (int l = 0; l < F.NL; ++l)
+    if (F.cm[l] && F.CC) {
+      const long long nv = (long long)N * F.CC * F.L[l].V;
+      hipLaunchKernelGGL((cm_permute<T, true>), dim3(nblocks(nv)), dim3(SV_BLOCK), 0, st, F.h[l], F.cm_h[l], F.CC, N,
+                         F.G[l], F.P[l].L);
+    }
   return 0;
 }
 

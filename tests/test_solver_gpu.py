@@ -116,6 +116,25 @@ def test_fmg_vs_oracle(sv, dt):
                 assert rel(x, ref) <= tol, (shape, p, bound)
 
 
+@pytest.mark.parametrize("case", [
+    ((64, 64, 64), 1, "dft", 3, torch.float32),      # 8 classes, K = 8 slots a row
+    ((64, 64, 64), 1, "dft", 3, torch.float64),
+    ((40, 48, 40), 4, "dct2", 2, torch.float32),     # 7 classes, partial last slot, batch of 4
+    ((64, 64, 64), 1, "dft", 1, torch.float32),      # membrane only: red/black
+])
+def test_fmg_class_major_vs_oracle(sv, case):
+    """Finest levels of at least 2^18 voxels (N V) relax in the class-major
+    layout: the FMG iterate equals the oracle's (same colour order)."""
+    shape, N, bound, ip, dt = case
+    rng = np.random.default_rng(40 + ip)
+    p = PENS[ip]
+    g, h = system(rng, shape, N=N)
+    x = sv.fmg_grid_cf(T(h, dt), T(g, dt), None, pen_vec(p), (1, 1, 1), bound, 2, 2)
+    ref = S.fmg_grid(h, g, bound=bound, nb_cycles=2, nb_iter=2, product_order=True, **p)
+    tol = 1e-4 if dt == torch.float32 else 1e-9
+    assert rel(x, ref) <= tol, (shape, N, bound, p)
+
+
 def test_relax_deterministic_with_wrap(sv):
     """Circulant axes whose length is not a multiple of the colour period:
     repeated relaxations give bit-identical results (no races)."""
