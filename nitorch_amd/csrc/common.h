@@ -4,15 +4,63 @@
 //    (nitorch/_C/src/bounds.h:7-17, semantics of bounds_common.h:28-219);
 //  * B-spline weight / derivative evaluation from the generated piece tables
 //    (semantics of interpolation_common.h:57-969);
-//  * error reporting for the C ABI (thread-local message, integer status).
+//  * error reporting for the C ABI (thread-local message, integer status);
+//  * stream-ordered scratch from a library-private memory pool.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cmath>
+#include <mutex>
 
 #include "spline_tables.h"
 
 namespace ni_amd {
+
+// --------------------------------------------------------------------------
+// Stream-ordered scratch.  Allocations come from a memory pool per device
+// owned by the library, which keeps up to SCRATCH_KEEP bytes reserved when
+// they are freed (the device's default pool releases freed memory at every
+// synchronisation, so each call would map it again: ~70 us a call, which
+// dominates small pulls and pushes).  Falls back to hipMallocAsync on the
+// default pool when a private pool cannot be made.
+// --------------------------------------------------------------------------
+#define SCRATCH_KEEP (uint64_t(256) << 20)
+
+inline hipMemPool_t scratch_pool() {
+  static std::mutex mu;
+  static hipMemPool_t pools[64] = {};
+  static bool tried[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!tried[dev]) {
+    tried[dev] = true;
+    hipMemPoolProps props = {};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = dev;
+    hipMemPool_t p = nullptr;
+    if (hipMemPoolCreate(&p, &props) == hipSuccess) {
+      uint64_t keep = SCRATCH_KEEP;
+      if (hipMemPoolSetAttribute(p, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess) pools[dev] = p;
+      else (void)hipMemPoolDestroy(p);
+    }
+    (void)hipGetLastError();
+  }
+  return pools[dev];
+}
+
+inline hipError_t scratch_alloc(void **ptr, size_t bytes, hipStream_t st) {
+  if (hipMemPool_t p = scratch_pool()) {
+    if (hipMallocFromPoolAsync(ptr, bytes, p, st) == hipSuccess) return hipSuccess;
+    (void)hipGetLastError();
+  }
+  return hipMallocAsync(ptr, bytes, st);
+}
 
 enum BoundCode : int {
   B_REPLICATE = 0, B_DCT1 = 1, B_DCT2 = 2, B_DST1 = 3, B_DST2 = 4,

@@ -413,7 +413,7 @@ static bool launch_binned(const PPArgs<float> &a, int ntx, int nty, int ntz, int
   const int64_t nsamp = ntiles * TS;   // listed samples, at most
   if (nbins >= (int64_t(1) << 31) || nsamp >= (int64_t(1) << 31)) return false;
   int *mem = nullptr;
-  if (hipMallocAsync((void **)&mem, sizeof(int) * (size_t)(2 * nbins + 2 + nsamp), st) != hipSuccess) {
+  if (scratch_alloc((void **)&mem, sizeof(int) * (size_t)(2 * nbins + 2 + nsamp), st) != hipSuccess) {
     (void)hipGetLastError();
     return false;
   }

@@ -20,12 +20,15 @@
 // The box has fixed dimensions per order (every tap offset is then an
 // immediate of the LDS instruction); channels are staged NCC at a time.
 // A tile whose box does not fit (scattered or far out-of-range
-// coordinates with extrapolation) is listed for a second kernel: tile_dyn
-// (pull / push / count) cuts it into sub-tiles with boxes sized to their own
-// spread, and runs the direct per-sample routine pp_sample for a sub-tile
-// that still does not fit; tile_fallback runs pp_sample for the gradient
-// modes.  The result never depends on the data's smoothness, only the speed
-// does.
+// coordinates with extrapolation) is listed for a second pass.  Pull, push
+// and count: in calls of at least BIN_MIN_SAMPLES samples the listed tiles'
+// samples are sorted by first-tap bin and every bin runs with a box of its
+// own (pushpull_bin.h); in smaller calls, and in the entry point that
+// allocates nothing, tile_dyn cuts each listed tile into sub-tiles with boxes
+// sized to their own spread and runs the direct per-sample routine pp_sample
+// for a sub-tile that still does not fit.  tile_fallback runs pp_sample for
+// the gradient modes.  The result never depends on the data's smoothness,
+// only the speed does.
 //
 // Weights, the out-of-bound rule and the splat order of each sample are
 // those of pp_sample (nitorch/_C/src/impl/pushpull_common.cpp: OOB :926-974,
@@ -41,6 +44,7 @@
 namespace ni_amd {
 
 #define TL_SLACK 2     // extra box extent per axis for the spread of a tile's coordinates
+#define BIN_MIN_SAMPLES (int64_t(1) << 20)   // smallest call whose listed tiles take the binned pass
 
 // Tile and box geometry of order ORD, fixed at compile time.
 //  * Gathers: 256 threads, 4 x 8 x 16 tiles, two samples per thread WY
@@ -1180,7 +1184,7 @@ static void launch_tile(const PPArgs<float> &a, hipStream_t st) {
   if (!own) {
     fb = a.scratch;   // the caller checked nb + 1 <= a.scratch_cap
     hipLaunchKernelGGL(fill_zero<int>, dim3(1), dim3(NI_BLOCK), 0, st, fb, (int64_t)1);
-  } else if (hipMallocAsync((void **)&fb, sizeof(int) * (size_t)(nb + 1), st) != hipSuccess) {
+  } else if (scratch_alloc((void **)&fb, sizeof(int) * (size_t)(nb + 1), st) != hipSuccess) {
     (void)hipGetLastError();
     fb = nullptr;
   } else {
@@ -1198,8 +1202,11 @@ static void launch_tile(const PPArgs<float> &a, hipStream_t st) {
   if (fb) {
     if constexpr (MODE == TM_PULL || MODE == TM_PUSH || MODE == TM_COUNT) {
       // listed tiles: binned by first tap (when the call may allocate its
-      // bins), else sub-tiles with boxes sized to their spread
-      if (!own || !launch_binned<ORD, MODE>(a, ntx, nty, ntz, nb, fb, st)) {
+      // bins and has at least BIN_MIN_SAMPLES samples: below that, the
+      // binned pass's six launches cost more than the sub-tile kernel's
+      // one), else sub-tiles with boxes sized to their spread
+      const bool big = a.N * a.tX * a.tY * a.tZ >= BIN_MIN_SAMPLES;
+      if (!own || !big || !launch_binned<ORD, MODE>(a, ntx, nty, ntz, nb, fb, st)) {
         constexpr int nsub = (B::TX / DYN_TX) * (B::TY / DYN_TY) * (B::TZ / DYN_TZ);
         const int64_t want = nb * nsub;
         const unsigned gd = (unsigned)(want < 4096 ? want : 4096);

@@ -358,7 +358,7 @@ static bool rs_separable(const T *input, T *output, const RsArgs &a, hipStream_t
     tot += ((long long)a.nt[axes[p]] * RS_MAXT * sizeof(int) + sizeof(T) - 1) / sizeof(T);
   }
   T *buf = nullptr;
-  if (hipMallocAsync((void **)&buf, (size_t)tot * sizeof(T), st) != hipSuccess) {
+  if (scratch_alloc((void **)&buf, (size_t)tot * sizeof(T), st) != hipSuccess) {
     (void)hipGetLastError();
     return false;
   }

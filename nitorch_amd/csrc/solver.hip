@@ -1436,7 +1436,7 @@ static size_t fmg_norms_offset(int N, int CC, int has_w, const Level *L, int NL,
 // g are permuted in and x out around each relax (buffers shared by the
 // levels), H once per solve (a buffer per level).  Sized for any linear
 // colouring with gcd(c, m) = 1, m <= 32: m ceil(Z / m) <= Z + 31 slots a row.
-#define CM_MIN_VOX (1 << 18)
+#define CM_MIN_VOX (1 << 16)
 static bool cm_level(const Level &L, int N) { return (long long)N * (long long)L.V >= CM_MIN_VOX; }
 static size_t cm_slots(const Level &L) { return (size_t)L.sz[0] * L.sz[1] * (L.sz[2] + 31); }
 static size_t cm_ws_elems(int N, int CC, const Level *L, int NL) {

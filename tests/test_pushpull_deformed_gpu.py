@@ -1,9 +1,11 @@
 """GPU parity of pull / push / count on strongly deformed grids, against the
 FP64 oracle: rotations with zoom (affine lattices) and smooth displacements
 with a large Jacobian, and samples collapsed at random into a small
-region.  Their tiles do not fit the static LDS box; their samples are
-binned by first tap (a few bins of the last grid hold thousands of
-samples, processed in batches).
+region.  Their tiles do not fit the static LDS box.  In calls of at least
+2^20 samples the listed tiles' samples are binned by first tap (a few bins
+of the collapsed grid hold thousands of samples, processed in batches); in
+smaller calls each listed tile is cut into sub-tiles with boxes of their
+own.  Both routes are tested.
 
 Tolerances: pull 1e-5 norm-relative (order 0: bit-exact); push / count
 element-wise |gpu - oracle| <= 1e-4 * oracle(|v|) (+ 1e-36, below float's
@@ -91,3 +93,33 @@ def test_deformed_pull_push_count(order):
                 out = G.grid_count(T(g), list(shape), [bound], [order], extrap).cpu().double().numpy()
                 ref = O.grid_count(g64, shape, bound, order, extrap, oob_f32=True)
                 assert (np.abs(out - ref) <= 1e-4 * ref + 1e-36).all(), ("count",) + case
+
+
+@pytest.mark.parametrize("order", [3])
+def test_deformed_binned_pass(order):
+    """The same checks at 100 x 104 x 104 (>= 2^20 samples: the binned second
+    pass)."""
+    from nitorch_amd import grid as G
+    rng = np.random.default_rng(1200 + order)
+    shape = (100, 104, 104)
+    for name, g in grids(rng, shape):
+        if name in ("rot35x_zoom0.8", "collapse6_far"):
+            continue
+        g = g.astype(np.float32)
+        g64 = g.astype(np.float64)
+        for bound in ("dct2", "dft"):
+            extrap = 1 if bound == "dft" else 0
+            case = (name, order, bound, extrap)
+            x = rng.standard_normal((1, 2) + shape).astype(np.float32)
+            out = G.grid_pull(T(x), T(g), [bound], [order], extrap).cpu().double().numpy()
+            ref = O.grid_pull(x.astype(np.float64), g64, bound, order, extrap, oob_f32=True)
+            assert np.abs(out - ref).max() <= 1e-5 * np.abs(ref).max(), ("pull",) + case
+            y = rng.standard_normal((1, 2) + shape).astype(np.float32)
+            y64 = y.astype(np.float64)
+            out = G.grid_push(T(y), T(g), list(shape), [bound], [order], extrap).cpu().double().numpy()
+            ref = O.grid_push(y64, g64, shape, bound, order, extrap, oob_f32=True)
+            bar = O.grid_push(np.abs(y64), g64, shape, bound, order, extrap, oob_f32=True)
+            assert (np.abs(out - ref) <= 1e-4 * bar + 1e-36).all(), ("push",) + case
+            out = G.grid_count(T(g), list(shape), [bound], [order], extrap).cpu().double().numpy()
+            ref = O.grid_count(g64, shape, bound, order, extrap, oob_f32=True)
+            assert (np.abs(out - ref) <= 1e-4 * ref + 1e-36).all(), ("count",) + case

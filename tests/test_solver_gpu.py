@@ -123,8 +123,9 @@ def test_fmg_vs_oracle(sv, dt):
     ((64, 64, 64), 1, "dft", 1, torch.float32),      # membrane only: red/black
 ])
 def test_fmg_class_major_vs_oracle(sv, case):
-    """Finest levels of at least 2^18 voxels (N V) relax in the class-major
-    layout: the FMG iterate equals the oracle's (same colour order)."""
+    """Levels of at least 2^16 voxels (N V) relax in the class-major layout
+    (here the finest level of each case): the FMG iterate equals the
+    oracle's (same colour order)."""
     shape, N, bound, ip, dt = case
     rng = np.random.default_rng(40 + ip)
     p = PENS[ip]
