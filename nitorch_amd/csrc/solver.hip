@@ -886,14 +886,29 @@ __global__ __launch_bounds__(SV_BLOCK) void pcg_matvec(const T *__restrict__ p, 
 #define MV_BZ (MV_TZ + 4)
 #define MV_BVOL (MV_BX * MV_BY * MV_BZ)
 static_assert(MV_TY * MV_TZ == SV_BLOCK, "one thread per (y, z) column of the tile");
+#define MV_MIN_VOX (1 << 16)   // residual and PCG start use the tiled operator from this many voxels (N V)
 
+// Float operator kernels on an LDS-staged box of the field (folded and
+// signed once), one thread per (y, z) column of a 4 x 8 x 32 tile:
+//  MV_APPLY: out = (H + L) u; pap[n][slot] += u.out   (pap may be null)
+//  MV_RESID: out = g - (H + L) u; acc[slot] += |out|^2, acc[nslot + slot] += |g|^2
+//            (acc may be null)
+//  MV_INIT : PCG start: out (r) = g - (H + L) u, z = (H + diag L)^-1 r, pz = z,
+//            pap[n][slot] += r.z
+// Partial sums go to slot (tile index mod the slot count), so the atomics of
+// neighbouring tiles spread over many addresses.
+enum MvMode { MV_APPLY = 0, MV_RESID = 1, MV_INIT = 2 };
+
+template <int MODE>
 __global__ __launch_bounds__(SV_BLOCK) void pcg_matvec_tile(const float *__restrict__ p, const float *__restrict__ h,
                                                             int CC, float *__restrict__ Ap, Geo G, StencilT<float> S,
-                                                            int ntx, int nty, int ntz, double *__restrict__ pap) {
+                                                            int ntx, int nty, int ntz, double *__restrict__ pap,
+                                                            const float *__restrict__ g, float *__restrict__ zo,
+                                                            float *__restrict__ po, int nslot) {
   __shared__ float box[3 * MV_BVOL];
   __shared__ int fofs[MV_BX + MV_BY + MV_BZ];
   __shared__ float fsg[MV_BX + MV_BY + MV_BZ];
-  __shared__ double s_acc[SV_BLOCK / 64];
+  __shared__ double s_acc[2][SV_BLOCK / 64];
   const int V = G.X * G.Y * G.Z;
   long long r = sv_block();
   const int tz = (int)(r % ntz); r /= ntz;
@@ -937,8 +952,9 @@ __global__ __launch_bounds__(SV_BLOCK) void pcg_matvec_tile(const float *__restr
   const int ly = threadIdx.x / MV_TZ, lz = threadIdx.x % MV_TZ;
   const int y = y0 + ly, z = z0 + lz;
   const float *hn = h ? h + (size_t)n * CC * V : nullptr;
-  float *An = Ap + (size_t)n * 3 * V;
-  double acc = 0.0;
+  const size_t b3 = (size_t)n * 3 * V;
+  float *An = Ap + b3;
+  double acc = 0.0, acc2 = 0.0;
 #pragma unroll 1
   for (int lx = 0; lx < MV_TX; ++lx) {
     const int x = x0 + lx;
@@ -954,18 +970,48 @@ __global__ __launch_bounds__(SV_BLOCK) void pcg_matvec_tile(const float *__restr
     const float a0 = Lp[0] + M[0] * p0 + M[3] * p1 + M[4] * p2;
     const float a1 = Lp[1] + M[3] * p0 + M[1] * p1 + M[5] * p2;
     const float a2 = Lp[2] + M[4] * p0 + M[5] * p1 + M[2] * p2;
-    An[iv] = a0; An[V + iv] = a1; An[2 * V + iv] = a2;
-    acc += (double)p0 * a0 + (double)p1 * a1 + (double)p2 * a2;
+    if (MODE == MV_APPLY) {
+      An[iv] = a0; An[V + iv] = a1; An[2 * V + iv] = a2;
+      acc += (double)p0 * a0 + (double)p1 * a1 + (double)p2 * a2;
+    } else {
+      const float g0 = g[b3 + iv], g1 = g[b3 + V + iv], g2 = g[b3 + 2 * V + iv];
+      const float r0 = g0 - a0, r1 = g1 - a1, r2 = g2 - a2;
+      An[iv] = r0; An[V + iv] = r1; An[2 * V + iv] = r2;
+      if (MODE == MV_RESID) {
+        acc += (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;
+        acc2 += (double)g0 * g0 + (double)g1 * g1 + (double)g2 * g2;
+      } else {
+        float q0, q1, q2;
+        block_jacobi<float>(hn, CC, V, iv, S.diag, r0, r1, r2, q0, q1, q2);
+        zo[b3 + iv] = q0; zo[b3 + V + iv] = q1; zo[b3 + 2 * V + iv] = q2;
+        po[b3 + iv] = q0; po[b3 + V + iv] = q1; po[b3 + 2 * V + iv] = q2;
+        acc += (double)r0 * q0 + (double)r1 * q1 + (double)r2 * q2;
+      }
+    }
   }
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if ((threadIdx.x & 63) == 0) s_acc[threadIdx.x >> 6] = acc;
+  if (!pap) return;   // uniform: no partial sums wanted
+  for (int off = 32; off > 0; off >>= 1) {
+    acc += __shfl_down(acc, off, 64);
+    if (MODE == MV_RESID) acc2 += __shfl_down(acc2, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_acc[0][threadIdx.x >> 6] = acc;
+    s_acc[1][threadIdx.x >> 6] = acc2;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double a = 0;
+    double a = 0, b = 0;
 #pragma unroll
-    for (int k = 0; k < SV_BLOCK / 64; ++k) a += s_acc[k];
-    const int slot = (int)(((long long)(tx * nty + ty) * ntz + tz) % SV_SLOTS);
-    if (a != 0) atomicAdd(pap + (size_t)n * SV_SLOTS + slot, a);
+    for (int k = 0; k < SV_BLOCK / 64; ++k) { a += s_acc[0][k]; b += s_acc[1][k]; }
+    const long long t = ((long long)(tx * nty + ty) * ntz + tz);
+    if (MODE == MV_RESID) {
+      const int slot = (int)(t % nslot);
+      atomicAdd(pap + slot, a);
+      atomicAdd(pap + nslot + slot, b);
+    } else {
+      const int slot = (int)(t % SV_SLOTS);
+      if (a != 0) atomicAdd(pap + (size_t)n * SV_SLOTS + slot, a);
+    }
   }
 }
 
@@ -1050,6 +1096,7 @@ __global__ __launch_bounds__(SV_BLOCK) void precond_kernel(const T *__restrict__
 // chscale: per output channel multiplier (displacement / Hessian rescaling).
 // accumulate: out += result (prolongation of a correction).
 // --------------------------------------------------------------------------
+#define XF_MIN_OUT (1 << 16)   // transfers with at least this many outputs use the LDS-tiled kernel
 struct Xfer {
   int Xi, Yi, Zi, Xo, Yo, Zo;
   double sc[3];
@@ -1117,6 +1164,127 @@ __global__ __launch_bounds__(SV_BLOCK) void transfer_kernel(const T *__restrict_
   else *dst = acc;
 }
 
+// First tap of output coordinate w: 4 linear taps (restriction) or 3
+// quadratic ones (prolongation), as transfer_kernel.
+template <int NT>
+__host__ __device__ inline int xfer_lo(int w, double sc) {
+  const double xc = (w + 0.5) * sc - 0.5;
+  return NT == 4 ? (int)floor(xc) - 1 : (int)floor(xc + 0.5) - 1;
+}
+
+// transfer_kernel on an LDS-staged box: a work-group owns a TX x TY x TZ
+// tile of one (n, c) output plane, stages the folded, signed input box its
+// taps reach (at most BX x BY x BZ; the host checks every tile's extent), and
+// each thread sums its outputs' NT^3 taps from LDS in transfer_kernel's order
+// with its weights (same values, bit for bit).  One global read of each box
+// element instead of one gather per tap and output.
+template <typename T, int NT, int TX, int TY, int TZ, int BX, int BY, int BZ>
+__global__ __launch_bounds__(SV_BLOCK) void transfer_tile(const T *__restrict__ in, T *__restrict__ out, int NC,
+                                                          int C, Xfer P, int accumulate, int ntx, int nty, int ntz) {
+  static_assert(TY * TZ == SV_BLOCK, "one thread per (y, z) column of the tile");
+  __shared__ T box[BX * BY * BZ];
+  __shared__ int fofs[BX + BY + BZ];
+  __shared__ T fsg[BX + BY + BZ];
+  long long r = sv_block();
+  const int tz = (int)(r % ntz); r /= ntz;
+  const int ty = (int)(r % nty); r /= nty;
+  const int tx = (int)(r % ntx);
+  const int nc = (int)(r / ntx);
+  const int so[3] = {P.Xo, P.Yo, P.Zo}, si[3] = {P.Xi, P.Yi, P.Zi};
+  const int ts[3] = {TX, TY, TZ}, w0[3] = {tx * TX, ty * TY, tz * TZ};
+  const int bmax[3] = {BX, BY, BZ}, stride[3] = {P.Yi * P.Zi, P.Zi, 1};
+  int lo0[3], ext[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    lo0[d] = xfer_lo<NT>(w0[d], P.sc[d]);
+    const int wl = min(w0[d] + ts[d], so[d]) - 1;
+    ext[d] = min(xfer_lo<NT>(wl, P.sc[d]) - lo0[d] + NT, bmax[d]);
+  }
+  for (int i = threadIdx.x; i < BX + BY + BZ; i += SV_BLOCK) {
+    const int d = i < BX ? 0 : (i < BX + BY ? 1 : 2);
+    const int b = i - (d == 0 ? 0 : (d == 1 ? BX : BX + BY));
+    int sg = 0, j = 0;
+    if (b < ext[d]) j = fold<int>(P.b[d], lo0[d] + b, si[d], sg);
+    fofs[i] = j * stride[d];
+    fsg[i] = T(sg);
+  }
+  __syncthreads();
+  // each thread stages whole x rows: all BX loads issued before any store
+  // (positions past the extent have offset 0 and sign 0: a valid address,
+  // staged as 0 and never read)
+  const T *src = in + (size_t)nc * P.Xi * P.Yi * P.Zi;
+  for (int q = threadIdx.x; q < BY * BZ; q += SV_BLOCK) {
+    const int by = q / BZ, bz = q - by * BZ;
+    if (by >= ext[1] || bz >= ext[2]) continue;
+    const int yzo = fofs[BX + by] + fofs[BX + BY + bz];
+    const T yzs = fsg[BX + by] * fsg[BX + BY + bz];
+    T v[BX];
+#pragma unroll
+    for (int bx = 0; bx < BX; ++bx) v[bx] = src[fofs[bx] + yzo];
+#pragma unroll
+    for (int bx = 0; bx < BX; ++bx) {
+      const T sg = fsg[bx] * yzs;
+      box[(bx * BY + by) * BZ + bz] = sg != T(0) ? v[bx] * sg : T(0);
+    }
+  }
+  __syncthreads();
+  const int ly = threadIdx.x / TZ, lz = threadIdx.x - ly * TZ;
+  const int y = w0[1] + ly, z = w0[2] + lz;
+  if (y >= P.Yo || z >= P.Zo) return;
+  T wy[NT], wz[NT];
+  int oy, oz;
+  {
+    const int w[2] = {y, z};
+    T *wt[2] = {wy, wz};
+    int *ob[2] = {&oy, &oz};
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int d = e + 1;
+      const double xc = (w[e] + 0.5) * P.sc[d] - 0.5;
+      const int lo = xfer_lo<NT>(w[e], P.sc[d]);
+#pragma unroll
+      for (int k = 0; k < NT; ++k) {
+        const double dd = xc - (double)(lo + k);
+        wt[e][k] = T(NT == 4 ? w_lin(dd / P.sc[d]) / P.sc[d] : w_quad(dd));
+      }
+      *ob[e] = lo - lo0[d];
+    }
+  }
+  T *dst = out + (size_t)nc * P.Xo * P.Yo * P.Zo;
+  const T cs = P.scaled ? T(P.chscale[nc % C]) : T(1);
+#pragma unroll 1
+  for (int lx = 0; lx < TX; ++lx) {
+    const int x = w0[0] + lx;
+    if (x >= P.Xo) break;
+    const double xc = (x + 0.5) * P.sc[0] - 0.5;
+    const int lo = xfer_lo<NT>(x, P.sc[0]);
+    T wx[NT];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+      const double dd = xc - (double)(lo + k);
+      wx[k] = T(NT == 4 ? w_lin(dd / P.sc[0]) / P.sc[0] : w_quad(dd));
+    }
+    const T *bp = box + ((lo - lo0[0]) * BY + oy) * BZ + oz;
+    T acc = T(0);
+#pragma unroll
+    for (int a = 0; a < NT; ++a) {
+      T accy = T(0);
+#pragma unroll
+      for (int b = 0; b < NT; ++b) {
+        T accz = T(0);
+#pragma unroll
+        for (int c = 0; c < NT; ++c) accz += bp[(a * BY + b) * BZ + c] * wz[c];
+        accy += accz * wy[b];
+      }
+      acc += accy * wx[a];
+    }
+    if (P.scaled) acc *= cs;
+    T *o = dst + ((size_t)x * P.Yo + y) * P.Zo + z;
+    if (accumulate) *o += acc;
+    else *o = acc;
+  }
+}
+
 template <typename T>
 __global__ void fill_kernel(T *p, long long n, T v) {
   long long i = (long long)blockIdx.x * SV_BLOCK + threadIdx.x;
@@ -1143,6 +1311,20 @@ template <typename T>
 static void launch_reg(const T *x, const T *h, int CC, const T *g, T *out, const Geo &G,
                        const StencilT<T> &S, int N, int mode, double *acc, hipStream_t st, int nslot = 1) {
   const long long n = (long long)N * G.X * G.Y * G.Z;
+  if constexpr (std::is_same<T, float>::value) {
+    if (!S.wmap && n >= MV_MIN_VOX) {   // the LDS-tiled operator
+      const int ntx = (G.X + MV_TX - 1) / MV_TX, nty = (G.Y + MV_TY - 1) / MV_TY, ntz = (G.Z + MV_TZ - 1) / MV_TZ;
+      const long long nb = (long long)N * ntx * nty * ntz;
+      if (mode == 1)
+        hipLaunchKernelGGL(pcg_matvec_tile<MV_RESID>, dim3((unsigned)nb), dim3(SV_BLOCK), 0, st, x, h, h ? CC : 0,
+                           out, G, S, ntx, nty, ntz, acc, g, (float *)nullptr, (float *)nullptr, nslot);
+      else
+        hipLaunchKernelGGL(pcg_matvec_tile<MV_APPLY>, dim3((unsigned)nb), dim3(SV_BLOCK), 0, st, x, h, h ? CC : 0,
+                           out, G, S, ntx, nty, ntz, (double *)nullptr, (const float *)nullptr, (float *)nullptr,
+                           (float *)nullptr, 0);
+      return;
+    }
+  }
   hipLaunchKernelGGL(reg_kernel<T>, dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, x, h, CC, g, out, G, S, N, mode, acc,
                      nslot);
 }
@@ -1321,10 +1503,43 @@ static void launch_transfer(const T *in, T *out, int N, int C, const int *si, co
   for (int c = 0; c < 6; ++c) P.chscale[c] = (chscale && c < C) ? chscale[c] : 1.0f;
   P.scaled = chscale != nullptr;   // FMG callers pass C <= 6 factors
   const long long n = (long long)N * C * so[0] * so[1] * so[2];
-  if (restrict_)
-    hipLaunchKernelGGL((transfer_kernel<T, 4>), dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, in, out, N * C, C, P, accumulate);
-  else
-    hipLaunchKernelGGL((transfer_kernel<T, 3>), dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, in, out, N * C, C, P, accumulate);
+  // the LDS-tiled kernel when every tile's input box fits its static box
+  // (FMG transfers: ratio <= 2 for restriction, <= 1 for prolongation)
+  auto fits = [&](int nt, const int *ts, const int *bm) {
+    for (int d = 0; d < 3; ++d)
+      for (int w0 = 0; w0 < so[d]; w0 += ts[d]) {
+        const int wl = std::min(w0 + ts[d], so[d]) - 1;
+        const int lo0 = nt == 4 ? xfer_lo<4>(w0, P.sc[d]) : xfer_lo<3>(w0, P.sc[d]);
+        const int lo1 = nt == 4 ? xfer_lo<4>(wl, P.sc[d]) : xfer_lo<3>(wl, P.sc[d]);
+        if (lo1 - lo0 + nt > bm[d]) return false;
+      }
+    return true;
+  };
+  if (restrict_) {
+    constexpr int TX = 2, TY = 8, TZ = 32, BX = 7, BY = 19, BZ = 67;
+    const int ts[3] = {TX, TY, TZ}, bm[3] = {BX, BY, BZ};
+    if (n >= XF_MIN_OUT && fits(4, ts, bm)) {
+      const int ntx = (so[0] + TX - 1) / TX, nty = (so[1] + TY - 1) / TY, ntz = (so[2] + TZ - 1) / TZ;
+      const long long nb = (long long)N * C * ntx * nty * ntz;
+      hipLaunchKernelGGL((transfer_tile<T, 4, TX, TY, TZ, BX, BY, BZ>), dim3((unsigned)nb), dim3(SV_BLOCK), 0, st, in,
+                         out, N * C, C, P, accumulate, ntx, nty, ntz);
+    } else {
+      hipLaunchKernelGGL((transfer_kernel<T, 4>), dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, in, out, N * C, C, P,
+                         accumulate);
+    }
+  } else {
+    constexpr int TX = 4, TY = 8, TZ = 32, BX = 7, BY = 11, BZ = 35;
+    const int ts[3] = {TX, TY, TZ}, bm[3] = {BX, BY, BZ};
+    if (n >= XF_MIN_OUT && fits(3, ts, bm)) {
+      const int ntx = (so[0] + TX - 1) / TX, nty = (so[1] + TY - 1) / TY, ntz = (so[2] + TZ - 1) / TZ;
+      const long long nb = (long long)N * C * ntx * nty * ntz;
+      hipLaunchKernelGGL((transfer_tile<T, 3, TX, TY, TZ, BX, BY, BZ>), dim3((unsigned)nb), dim3(SV_BLOCK), 0, st, in,
+                         out, N * C, C, P, accumulate, ntx, nty, ntz);
+    } else {
+      hipLaunchKernelGGL((transfer_kernel<T, 3>), dim3(nblocks(n)), dim3(SV_BLOCK), 0, st, in, out, N * C, C, P,
+                         accumulate);
+    }
+  }
 }
 
 // PCG scratch: r, z, p (3 fields each) then 4 N SV_SLOTS doubles (slots of the
@@ -1347,8 +1562,8 @@ static void launch_matvec(const T *p, const T *h, int CC, T *Ap, const Geo &G, c
   if constexpr (std::is_same<T, float>::value) {
     const int ntx = (G.X + MV_TX - 1) / MV_TX, nty = (G.Y + MV_TY - 1) / MV_TY, ntz = (G.Z + MV_TZ - 1) / MV_TZ;
     const long long nb = (long long)N * ntx * nty * ntz;
-    hipLaunchKernelGGL(pcg_matvec_tile, dim3((unsigned)nb), dim3(SV_BLOCK), 0, st, p, h, CC, Ap, G, S, ntx, nty,
-                       ntz, pap);
+    hipLaunchKernelGGL(pcg_matvec_tile<MV_APPLY>, dim3((unsigned)nb), dim3(SV_BLOCK), 0, st, p, h, CC, Ap, G, S,
+                       ntx, nty, ntz, pap, (const float *)nullptr, (float *)nullptr, (float *)nullptr, 0);
   } else {
     hipLaunchKernelGGL(pcg_matvec<T>, dim3(bv), dim3(SV_BLOCK), 0, st, p, h, CC, Ap, G, S, N, pap);
   }
@@ -1363,7 +1578,18 @@ static int run_pcg(T *x, const T *g, const T *h, int CC, const Geo &G, const Ste
   double *pap = sc + 3 * q;
   launch_fill<double>(sc, 4LL * (long long)q, 0.0, st);
   const unsigned bv = nblocks((long long)N * V), bf = nblocks((long long)N * 3 * V);
-  hipLaunchKernelGGL(pcg_init<T>, dim3(bv), dim3(SV_BLOCK), 0, st, x, g, h, CC, r, z, p, G, S, N, R[0]);
+  bool tiled = false;
+  if constexpr (std::is_same<T, float>::value) {
+    if (!S.wmap && (long long)N * V >= MV_MIN_VOX) {   // the LDS-tiled operator
+      const int ntx = (G.X + MV_TX - 1) / MV_TX, nty = (G.Y + MV_TY - 1) / MV_TY, ntz = (G.Z + MV_TZ - 1) / MV_TZ;
+      const long long nb = (long long)N * ntx * nty * ntz;
+      hipLaunchKernelGGL(pcg_matvec_tile<MV_INIT>, dim3((unsigned)nb), dim3(SV_BLOCK), 0, st, x, h, h ? CC : 0, r, G,
+                         S, ntx, nty, ntz, R[0], g, z, p, 0);
+      tiled = true;
+    }
+  }
+  if (!tiled)
+    hipLaunchKernelGGL(pcg_init<T>, dim3(bv), dim3(SV_BLOCK), 0, st, x, g, h, CC, r, z, p, G, S, N, R[0]);
   int it = 0;
   std::vector<double> host(2 * q);
   for (; it < nb_iter; ++it) {

@@ -104,6 +104,23 @@ def test_transfers_vs_oracle(sv, dt):
 
 
 @pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_transfers_tiled_vs_oracle(sv, dt):
+    """Transfers of at least 2^16 outputs run the LDS-tiled kernel (odd and
+    even sizes, folded and signed bounds)."""
+    rng = np.random.default_rng(13)
+    tol = 1e-6 if dt == torch.float32 else 1e-12
+    for shape in [(64, 50, 45), (48, 64, 33)]:
+        for bound in ("dct2", "dft", "dst2", "zero"):
+            x = rng.standard_normal((2, 3) + shape)
+            cs = tuple(1 + (s - 1) // 2 for s in shape)
+            out = sv.fmg_restriction(T(x, dt), torch.empty((2, 3) + cs, dtype=dt, device=DEV), bound)
+            assert rel(out, S.restrict(x, cs, bound)) <= tol, ("restrict", shape, bound)
+            xc = rng.standard_normal((2, 3) + cs)
+            out = sv.fmg_prolongation(T(xc, dt), torch.empty((2, 3) + shape, dtype=dt, device=DEV), bound)
+            assert rel(out, S.prolong(xc, shape, bound)) <= tol, ("prolong", shape, bound)
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
 def test_fmg_vs_oracle(sv, dt):
     rng = np.random.default_rng(4)
     for shape in [(12, 10, 9), (24, 20, 18)]:
