@@ -749,6 +749,8 @@ __device__ __forceinline__ void voxel_diag(const Geo &G, const StencilT<T> &S, i
 // V >= SV_BLOCK (both are reduced in the block); smaller levels fall back to
 // per-thread atomics.  Summation order is that of the atomics (FP64).
 #define SV_SLOTS 256
+#define PD_CHUNK 4096   // elements a work-group of pcg_direction updates
+#define PD_MIN_ELEMS (1 << 22)   // fields from this size take the chunked pcg_direction (fewer work-groups)
 
 __device__ __forceinline__ void batch_reduce(double acc, int n, int V, double *__restrict__ out) {
   const int slot = (int)(sv_block() & (SV_SLOTS - 1));
@@ -1059,6 +1061,40 @@ __global__ __launch_bounds__(SV_BLOCK) void pcg_direction(T *__restrict__ p, con
                                                           const double *__restrict__ rz,
                                                           const double *__restrict__ rz_new,
                                                           double *__restrict__ clear_a, double *__restrict__ clear_b) {
+  const long long gid = sv_block() * SV_BLOCK + threadIdx.x;
+  for (long long k = gid; k < (long long)N * SV_SLOTS; k += (long long)gridDim.x * SV_BLOCK) {
+    clear_a[k] = 0;
+    clear_b[k] = 0;
+  }
+  // a block owns PD_CHUNK consecutive elements (PD_CHUNK / SV_BLOCK a thread,
+  // coalesced), so that the per-block sums of the r.z slots (block_scalars)
+  // are paid once per chunk; a chunk spans at most two batch elements
+  // (the host uses this kernel when N 3 V >= PD_MIN_ELEMS and 3 V >= PD_CHUNK)
+  BlockScalars bs;
+  block_scalars(rz_new, rz, 3 * V, N, PD_CHUNK, bs);
+  const long long c0 = sv_block() * PD_CHUNK, tot = (long long)N * 3 * V;
+  T beta[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const double den = bs.b[j] > 1e-12 ? bs.b[j] : 1e-12;
+    beta[j] = T(bs.a[j] / den);
+  }
+#pragma unroll 4
+  for (int k = 0; k < PD_CHUNK / SV_BLOCK; ++k) {
+    const long long e = c0 + (long long)k * SV_BLOCK + threadIdx.x;
+    if (e >= tot) break;
+    const int n = (int)(e / (3LL * V));
+    p[e] = z[e] + beta[n == bs.n0 ? 0 : 1] * p[e];
+  }
+}
+
+// the same, one element a thread (smaller fields: enough work-groups to hide
+// the loads' latency)
+template <typename T>
+__global__ __launch_bounds__(SV_BLOCK) void pcg_direction1(T *__restrict__ p, const T *__restrict__ z, int V, int N,
+                                                           const double *__restrict__ rz,
+                                                           const double *__restrict__ rz_new,
+                                                           double *__restrict__ clear_a, double *__restrict__ clear_b) {
   const long long gid = sv_block() * SV_BLOCK + threadIdx.x;
   for (long long k = gid; k < (long long)N * SV_SLOTS; k += (long long)gridDim.x * SV_BLOCK) {
     clear_a[k] = 0;
@@ -1612,7 +1648,13 @@ static int run_pcg(T *x, const T *g, const T *h, int CC, const Geo &G, const Ste
       if (obj < tol * (double)N * (double)V) break;
     }
     hipLaunchKernelGGL(pcg_update<T>, dim3(bv), dim3(SV_BLOCK), 0, st, x, r, z, p, h, CC, G, S, N, cur, pap, nxt);
-    hipLaunchKernelGGL(pcg_direction<T>, dim3(bf), dim3(SV_BLOCK), 0, st, p, z, (int)V, N, cur, nxt, spare, pap);
+    if (3 * V >= PD_CHUNK && (long long)N * 3 * V >= PD_MIN_ELEMS) {
+      const long long nchunk = ((long long)N * 3 * V + PD_CHUNK - 1) / PD_CHUNK;
+      hipLaunchKernelGGL(pcg_direction<T>, dim3((unsigned)nchunk), dim3(SV_BLOCK), 0, st, p, z, (int)V, N, cur, nxt,
+                         spare, pap);
+    } else {
+      hipLaunchKernelGGL(pcg_direction1<T>, dim3(bf), dim3(SV_BLOCK), 0, st, p, z, (int)V, N, cur, nxt, spare, pap);
+    }
   }
   if (iters) *iters = it;
   return 0;

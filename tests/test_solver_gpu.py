@@ -250,6 +250,23 @@ def test_pcg_vs_oracle(sv, dt):
                     assert rel(x, ref) <= tol, (shape, p, bound, CC, N)
 
 
+def test_pcg_large_vs_oracle(sv):
+    """Fields of at least 2^22 elements (the LDS-tiled PCG start and
+    operator, the chunked direction update; batch of 2, so a chunk can span
+    two batch elements), both dtypes against one oracle solve."""
+    rng = np.random.default_rng(19)
+    shape = (64, 96, 120)
+    p = PENS[3]
+    g, h = system(rng, shape, 6, 2)
+    x0 = rng.standard_normal(g.shape) * 0.1
+    ref, _ = S.pcg_grid(h, g, x0, voxel_size=(1, 1.2, 0.9), bound="dft", nb_iter=4, **p)
+    for dt, tol in ((torch.float32, 1e-4), (torch.float64, 1e-9)):
+        x = T(x0, dt)
+        _, it = sv.pcg_grid_cf(T(h, dt), T(g, dt), x, pen_vec(p), (1, 1.2, 0.9), "dft", nb_iter=4)
+        assert it == 4
+        assert rel(x, ref) <= tol, dt
+
+
 def test_pcg_early_stop_vs_oracle(sv):
     """tol > 0: same stopping iteration and iterate as the oracle (FP64)."""
     rng = np.random.default_rng(10)
