@@ -261,3 +261,49 @@ def test_label_map_pull_vs_oracle(prefilter):
 
 def O_identity(shape):
     return np.stack(np.meshgrid(*[np.arange(s, dtype=np.float64) for s in shape], indexing="ij"), -1)
+
+
+def test_int64_offsets():
+    """Tensors whose largest element offset does not fit 32 bits take the
+    64-bit-offset direct kernel (the reference's int64 dispatch,
+    pushpull_common.cpp:2506-2519).  Here the source and the pushed values are
+    two small volumes 2^31 elements apart in one 8 GiB buffer (batch stride
+    2^31): pull, grid_grad and push must equal the oracle -- order 0
+    bit-exactly -- as on contiguous copies, which take the 32-bit path."""
+    from nitorch_amd import grid as G
+    rng = np.random.default_rng(77)
+    shape = (9, 10, 11)
+    V, S = 990, 2 ** 31
+    big = torch.zeros(S + 4 * 65536, device=DEV)
+    strides = (S, V, 110, 11, 1)
+    x = rng.standard_normal((2, 2) + shape).astype(np.float32)
+    y = rng.standard_normal((2, 2) + shape).astype(np.float32)
+    src = big.as_strided((2, 2) + shape, strides, 0)
+    val = big.as_strided((2, 2) + shape, strides, 65536)
+    src.copy_(torch.from_numpy(x))
+    val.copy_(torch.from_numpy(y))
+    g = identity(shape).repeat(2, 0) + rng.uniform(-1.5, 1.5, (2,) + shape + (3,))
+    g = g.astype(np.float32)
+    g64, x64, y64 = g.astype(np.float64), x.astype(np.float64), y.astype(np.float64)
+    try:
+        for order in (0, 1, 3):
+            for bound in ("dct2", "dft", "zero"):
+                case = (order, bound)
+                out = G.grid_pull(src, T(g), [bound], [order], 0).cpu().double().numpy()
+                ref = O.grid_pull(x64, g64, bound, order, 0, oob_f32=True)
+                if order == 0:
+                    assert np.array_equal(out, ref), ("pull",) + case
+                assert np.abs(out - ref).max() <= 1e-5 * np.abs(ref).max(), ("pull",) + case
+                out32 = G.grid_pull(T(x), T(g), [bound], [order], 0).cpu().double().numpy()
+                assert np.abs(out - out32).max() <= 1e-5 * np.abs(ref).max(), ("pull 32 vs 64",) + case
+                if order > 0:
+                    out = G.grid_grad(src, T(g), [bound], [order], 0).cpu().double().numpy()
+                    ref = O.grid_grad(x64, g64, bound, order, 0, oob_f32=True)
+                    assert np.abs(out - ref).max() <= 1e-5 * np.abs(ref).max(), ("grad",) + case
+                out = G.grid_push(val, T(g), list(shape), [bound], [order], 0).cpu().double().numpy()
+                ref = O.grid_push(y64, g64, shape, bound, order, 0, oob_f32=True)
+                bar = O.grid_push(np.abs(y64), g64, shape, bound, order, 0, oob_f32=True)
+                assert (np.abs(out - ref) <= 1e-4 * bar + 1e-36).all(), ("push",) + case
+    finally:
+        del src, val, big
+        torch.cuda.empty_cache()
