@@ -13,10 +13,15 @@
 //            the rate of its 64-bit integer atomics, and ~40x faster than
 //            ds_add_f32), then add the box to the output once, folded and
 //            signed, with one FP32 global atomic per non-zero box element.
-//            Every box element is a double sum of its splats, so each output
-//            element keeps at least float's relative accuracy however small
-//            it is next to the tile's other values, and NaN / Inf values
-//            propagate as in the reference's float sums.
+//            At orders 1-3 a thread's z-consecutive samples whose taps are
+//            z-consecutive too first sum their splats along z in registers
+//            (at most 4 FP32 terms a box element) and add one FP64 atomic a
+//            position.  Every box element is a double sum of its splats
+//            (FP64 products; at orders 1-3, FP32 sums of at most 4 FP32
+//            products), so each output element keeps float's relative
+//            accuracy against the sum of its splats' magnitudes however
+//            small it is next to the tile's other values, and NaN / Inf
+//            values propagate as in the reference's float sums.
 // The box has fixed dimensions per order (every tap offset is then an
 // immediate of the LDS instruction); channels are staged NCC at a time.
 // A tile whose box does not fit (scattered or far out-of-range
@@ -99,8 +104,10 @@ struct TileBox {
   // and 1.75 ms at 4 channels)
   static constexpr int MAX_NCC = SCAT ? 1 : 4;
   // waves per SIMD asked of the register allocator for the 512-thread
-  // scatter: 6 (<= 85 VGPRs) lets three 52 KB boxes share a CU's LDS
-  static constexpr int MINW = BIG ? 6 : 1;
+  // scatter: 6 (<= 85 VGPRs) lets three 52 KB boxes share a CU's LDS; the
+  // windowed scatter (orders 1-3) holds its samples' weights and needs ~98
+  // VGPRs: two work-groups a CU
+  static constexpr int MINW = BIG ? (ORD >= 1 ? 4 : 6) : 1;
   static_assert(TX * TY * TZ == NTH * R, "R samples per thread");
   static_assert(WX * RX == TX && WX * WY == NW, "waves tile the x rows");
   static_assert(SCAT ? WY * RY == TY : WY * R == TY, "waves tile the y rows");
@@ -307,13 +314,23 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
   tl_thread<B>(sx, sy, sz0);
   const int px = tx * B::TX + sx, py = ty * B::TY + sy, pz0 = tz * B::TZ + sz0;
   bool valid[R], inb[R];
-  // pull keeps every sample's weights; the other modes keep the
-  // coordinates and compute one sample's weights (and derivatives) when it
-  // is processed (registers: occupancy)
-  constexpr int NW = (DERIV || !GATHER) ? 1 : R;
+  // pull and the windowed scatter keep every sample's weights; the other
+  // modes keep the coordinates and compute one sample's weights (and
+  // derivatives) when it is processed (registers: occupancy)
+  constexpr bool WIN = !GATHER && B::BIG && ORD >= 1;
+  static_assert(!WIN || (B::SDZ == 1 && B::SDY == 0), "the scatter window needs z-consecutive samples");
+  constexpr int NW = DERIV ? 1 : (GATHER || WIN) ? R : 1;
   int lo[R][3];
   float w[NW][3][NT];
   float xq[R][3];
+  if constexpr (WIN) {   // the window multiplies the weights of samples outside a group by 0: keep them finite
+#pragma unroll
+    for (int i = 0; i < NW; ++i)
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) w[i][d][t] = 0.f;
+  }
   int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
   const float tol = (a.extrap & 2) ? 0.55f : 0.05f;
   const int gD = (int)a.grid_s[4];
@@ -730,6 +747,63 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
         og[R - 1][2] = o2;
       }
     } else {
+      if constexpr (WIN) {
+      // push / count, orders 1-3: a thread's R z-consecutive samples whose
+      // first taps share x and y and advance by one along z with the sample
+      // share a window of R + NT - 1 z positions per (x, y) tap row.  Their
+      // splats are summed in registers (FP32, at most R terms a position:
+      // sample i reaches positions i .. i + NT - 1 of the window) and each
+      // position is added to the box by one FP64 atomic: R + NT - 1 atomics
+      // a row instead of R * NT.  Sample i's taps start at box element
+      // off[i], so samples with equal off[i] - i are exactly those one
+      // window at base off[i] - i serves.  A thread's samples split into
+      // such groups (anchored at its first sample left); the lanes of a wave
+      // loop until every group is done, so any grid is handled, a smooth
+      // one in one pass.  A non-finite value reaches only its own taps.
+        constexpr int WS = R + NT - 1;
+        double *dbox = (double *)box;
+        unsigned rem = 0;
+#pragma unroll
+        for (int i = 0; i < R; ++i) rem |= inb[i] ? 1u << i : 0u;
+        while (rem) {
+          const int an = __builtin_ctz(rem);
+          int base = off[0];
+#pragma unroll
+          for (int i = 1; i < R; ++i)
+            if (an == i) base = off[i] - i;
+          unsigned gm = 0;
+#pragma unroll
+          for (int i = 0; i < R; ++i)
+            if (((rem >> i) & 1u) && off[i] - i == base) gm |= 1u << i;
+          rem &= ~gm;
+          const int hi = 31 - __builtin_clz(gm) + NT - 1;   // last window position reached
+          // window position s is box element base + s (an <= s <= hi)
+          double *bp = dbox + base;
+#pragma unroll 1
+          for (int kx = 0; kx < NT; ++kx, bp += DYZ) {
+            float xv[R];
+#pragma unroll
+            for (int i = 0; i < R; ++i) xv[i] = ((gm >> i) & 1u) ? w[i][0][0] * tv[i][0] : 0.f;
+#pragma unroll
+            for (int ky = 0; ky < NT; ++ky) {
+              float acc[WS];
+#pragma unroll
+              for (int s = 0; s < WS; ++s) acc[s] = 0.f;
+#pragma unroll
+              for (int i = 0; i < R; ++i) {
+                const float cw = xv[i] * w[i][1][ky];
+#pragma unroll
+                for (int kz = 0; kz < NT; ++kz) acc[i + kz] = fmaf(cw, w[i][2][kz], acc[i + kz]);
+              }
+#pragma unroll
+              for (int s = 0; s < WS; ++s)
+                if (s >= an && s <= hi) atomicAdd(bp + ky * DZ + s, (double)acc[s]);
+            }
+#pragma unroll
+            for (int i = 0; i < R; ++i) tl_rot(w[i][0]);
+          }
+        }
+      } else {
       // push / count: FP64 splats, one sample at a time (rolled): element 0
       // of the per-sample state is the current sample, rotated after each
       // (R rotations restore it for the next channel chunk)
@@ -786,6 +860,7 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
           }
           tl_rot(wi[0]);
         }
+      }
       }
 
       // ---- add the box to the output, folded and signed; clear it for
