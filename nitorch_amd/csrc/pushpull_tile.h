@@ -13,11 +13,11 @@
 //            the rate of its 64-bit integer atomics, and ~40x faster than
 //            ds_add_f32), then add the box to the output once, folded and
 //            signed, with one FP32 global atomic per non-zero box element.
-//            At orders 1-3 a thread's z-consecutive samples whose taps are
+//            At orders >= 1 a thread's z-consecutive samples whose taps are
 //            z-consecutive too first sum their splats along z in registers
-//            (at most 4 FP32 terms a box element) and add one FP64 atomic a
-//            position.  Every box element is a double sum of its splats
-//            (FP64 products; at orders 1-3, FP32 sums of at most 4 FP32
+//            (at most R = 4 or 2 FP32 terms a box element) and add one FP64
+//            atomic a position.  Every box element is a double sum of its splats
+//            (FP64 products at order 0; above, FP32 sums of at most R FP32
 //            products), so each output element keeps float's relative
 //            accuracy against the sum of its splats' magnitudes however
 //            small it is next to the tile's other values, and NaN / Inf
@@ -105,8 +105,8 @@ struct TileBox {
   static constexpr int MAX_NCC = SCAT ? 1 : 4;
   // waves per SIMD asked of the register allocator for the 512-thread
   // scatter: 6 (<= 85 VGPRs) lets three 52 KB boxes share a CU's LDS; the
-  // windowed scatter (orders 1-3) holds its samples' weights and needs ~98
-  // VGPRs: two work-groups a CU
+  // windowed scatter (orders 1-3 here) holds its samples' weights and
+  // needs ~97 VGPRs: two work-groups a CU
   static constexpr int MINW = BIG ? (ORD >= 1 ? 4 : 6) : 1;
   static_assert(TX * TY * TZ == NTH * R, "R samples per thread");
   static_assert(WX * RX == TX && WX * WY == NW, "waves tile the x rows");
@@ -317,7 +317,7 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
   // pull and the windowed scatter keep every sample's weights; the other
   // modes keep the coordinates and compute one sample's weights (and
   // derivatives) when it is processed (registers: occupancy)
-  constexpr bool WIN = !GATHER && B::BIG && ORD >= 1;
+  constexpr bool WIN = !GATHER && ORD >= 1;
   static_assert(!WIN || (B::SDZ == 1 && B::SDY == 0), "the scatter window needs z-consecutive samples");
   constexpr int NW = DERIV ? 1 : (GATHER || WIN) ? R : 1;
   int lo[R][3];
@@ -748,7 +748,7 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
       }
     } else {
       if constexpr (WIN) {
-      // push / count, orders 1-3: a thread's R z-consecutive samples whose
+      // push / count, orders >= 1: a thread's R z-consecutive samples whose
       // first taps share x and y and advance by one along z with the sample
       // share a window of R + NT - 1 z positions per (x, y) tap row.  Their
       // splats are summed in registers (FP32, at most R terms a position:

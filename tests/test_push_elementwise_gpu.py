@@ -68,7 +68,7 @@ def _check(out, ref, bar, what):
     assert not bad.any(), (what, int(bad.sum()), float((err / np.maximum(bar, 1e-300))[bad].max()))
 
 
-@pytest.mark.parametrize("order", [1, 2, 3, 5])
+@pytest.mark.parametrize("order", [1, 2, 3, 5, 7])
 def test_push_elementwise_mixed_magnitudes(order):
     rng = np.random.default_rng(900 + order)
     shape = (20, 35, 33)
@@ -104,7 +104,7 @@ def test_push_elementwise_tiny_regions(order):
             _check(out, ref, bar, ("push", order, bound, extrap))
 
 
-@pytest.mark.parametrize("order", [0, 1, 3])
+@pytest.mark.parametrize("order", [0, 1, 3, 5, 7])
 def test_push_propagates_nan_and_inf(order):
     """One NaN and one +Inf among finite values: the output is NaN / Inf at
     exactly the elements where the oracle's is, and meets the element-wise
