@@ -54,12 +54,14 @@ namespace ni_amd {
 // Tile and box geometry of order ORD, fixed at compile time.
 //  * Gathers: 256 threads, 4 x 8 x 16 tiles, two samples per thread WY
 //    y-rows apart (larger tiles re-read less halo but cost occupancy, which
-//    the gather needs more).  Scatters up to order 3: 512 threads,
+//    the gather needs more).  Scatters up to order 5: 512 threads,
 //    8 x 16 x 16 tiles, four z-consecutive samples per thread -- every box
 //    element is one global atomic in the flush and a box exceeds its tile
 //    by the ORD + 1 taps plus TL_SLACK voxels per axis, so the atomics per
-//    sample fall with the tile size.  Scatters above order 3 (twice the
-//    weight registers): 256 threads, 4 x 8 x 16, two samples per thread.
+//    sample fall with the tile size, and the register window of the splat
+//    spans four samples.  Scatters above order 5 (the window's weights of
+//    four samples would not fit the registers, a box of 94 KB one CU's LDS
+//    twice): 256 threads, 4 x 8 x 16, two samples per thread.
 //  * Lanes: gathers put 16 z-samples x 4 x-rows in a wave; scatters
 //    (ZL = TZ / R threads per z-row) R y-rows fastest, then x-rows.
 //  * The box is [c][x][y][z] with x stride DYZP padded to 48 (mod 64) words
@@ -70,7 +72,7 @@ namespace ni_amd {
 //    lanes R banks apart per row).
 template <int ORD, bool SCAT>
 struct TileBox {
-  static constexpr bool BIG = SCAT && ORD <= 3;
+  static constexpr bool BIG = SCAT && ORD <= 5;   // the 512-thread scatter
   static constexpr int NTH = BIG ? 512 : 256;          // threads per work-group
   static constexpr int NW = NTH / 64;                  // waves
   static constexpr int R = BIG ? 4 : 2;                // samples per thread
@@ -105,8 +107,8 @@ struct TileBox {
   static constexpr int MAX_NCC = SCAT ? 1 : 4;
   // waves per SIMD asked of the register allocator for the 512-thread
   // scatter: 6 (<= 85 VGPRs) lets three 52 KB boxes share a CU's LDS; the
-  // windowed scatter (orders 1-3 here) holds its samples' weights and
-  // needs ~97 VGPRs: two work-groups a CU
+  // windowed scatter (orders 1-5 here) holds its samples' weights and
+  // needs 97 (order 3) to 128 (order 5) VGPRs: two work-groups a CU
   static constexpr int MINW = BIG ? (ORD >= 1 ? 4 : 6) : 1;
   static_assert(TX * TY * TZ == NTH * R, "R samples per thread");
   static_assert(WX * RX == TX && WX * WY == NW, "waves tile the x rows");
