@@ -287,15 +287,15 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
   constexpr bool GATHER = TileMeta<MODE>::gather, DERIV = TileMeta<MODE>::deriv;
   using B = TileBox<ORD, !GATHER>;
   constexpr int R = B::R;
-  // pull stages its box with the NCC = 4 channels of a voxel interleaved
-  // (one 16-byte LDS read per tap: gfx950's LDS serves 128-bit reads at
-  // ~1.5x the bytes per clock of 32-bit ones, and 16 z-consecutive lanes
-  // read 16 consecutive 16-byte voxels, which no bank serves twice);
-  // the other modes keep channel planes [c][x][y][z]
-  constexpr bool IL = MODE == TM_PULL;
+  // pull and grid_grad stage their box with the NCC = 4 channels of a
+  // voxel interleaved (one 16-byte LDS read per tap: gfx950's LDS serves
+  // 128-bit reads at ~1.5x the bytes per clock of 32-bit ones, and 16
+  // z-consecutive lanes read 16 consecutive 16-byte voxels, which no bank
+  // serves twice); the other modes keep channel planes [c][x][y][z]
+  constexpr bool IL = MODE == TM_PULL || (MODE == TM_SGRAD && NCC == 4);
   constexpr int DY = B::DY, DZ = B::DZ, DYZ = IL ? B::DYZ4 : B::DYZP, VOL = IL ? B::VOL4 : B::VOL;
   static_assert(NCC >= 1 && NCC <= B::MAX_NCC, "channel chunk exceeds the LDS budget");
-  static_assert(!IL || NCC == 4 || NCC == 1, "pull interleaves 4 channels (or runs one)");
+  static_assert(!IL || NCC == 4 || NCC == 1, "pull / grad interleave 4 channels (pull may run one)");
   constexpr int NTH = B::NTH;
   // gathers: FP32 source box; scatters: FP64 box (two words per element)
   __shared__ __align__(16) float box[(GATHER ? 1 : 2) * NCC * VOL];
@@ -647,18 +647,40 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
               tl_opaque(wi[0]);
               tl_opaque(gi[0]);
               const float wyz = wi[1][0] * wi[2][0], gyz = gi[1][0] * wi[2][0], wgz = wi[1][0] * gi[2][0];
-#pragma unroll
-              for (int c = 0; c < NCC; ++c) {
-                float sw = 0.f, sd = 0.f;
+              if constexpr (IL) {   // 4 interleaved channels: one 16-byte read a tap
+                // bq - box counts voxels here (off[] and the strides are in voxels)
+                const float4 *vq = (const float4 *)box + (bq - box);
+                float sw[4] = {0.f, 0.f, 0.f, 0.f}, sd[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int kx = 0; kx < NT; ++kx) {
-                  const float v = bq[c * VOL + kx * DYZ];
-                  sw += wi[0][kx] * v;
-                  sd += gi[0][kx] * v;
+                  const float4 v = vq[kx * DYZ];
+                  const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                  for (int c = 0; c < 4; ++c) {
+                    sw[c] += wi[0][kx] * vv[c];
+                    sd[c] += gi[0][kx] * vv[c];
+                  }
                 }
-                ax[c] += sd * wyz;
-                ay[c] += sw * gyz;
-                az[c] += sw * wgz;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                  ax[c] += sd[c] * wyz;
+                  ay[c] += sw[c] * gyz;
+                  az[c] += sw[c] * wgz;
+                }
+              } else {
+#pragma unroll
+                for (int c = 0; c < NCC; ++c) {
+                  float sw = 0.f, sd = 0.f;
+#pragma unroll
+                  for (int kx = 0; kx < NT; ++kx) {
+                    const float v = bq[c * VOL + kx * DYZ];
+                    sw += wi[0][kx] * v;
+                    sd += gi[0][kx] * v;
+                  }
+                  ax[c] += sd * wyz;
+                  ay[c] += sw * gyz;
+                  az[c] += sw * wgz;
+                }
               }
               tl_rot(wi[1]);
               tl_rot(gi[1]);
