@@ -100,10 +100,9 @@ struct TileBox {
   static constexpr int VOL4 = DX * DYZ4;
   static constexpr int NPP = (DYZ + NTH - 1) / NTH;     // y-z plane positions per thread
   // channels staged at once: 1 for the scatters (a 52 KB box of doubles
-  // at order 3) and 2 for pull / grid gradient, 4 for grid_grad (its 3 accumulators per
-  // channel amortise the taps' weights and derivatives over more channels:
-  // 4x8x192^3, order 3, 3.13 -> 2.67 ms, while pull itself is 1.57 ms at 2
-  // and 1.75 ms at 4 channels)
+  // at order 3), 4 for the gathers (pull, grid_grad, the grid gradients of
+  // the backward passes), interleaved per voxel so that a tap is one
+  // 16-byte LDS read
   static constexpr int MAX_NCC = SCAT ? 1 : 4;
   // waves per SIMD asked of the register allocator for the 512-thread
   // scatter: 6 (<= 85 VGPRs) lets three 52 KB boxes share a CU's LDS; the
@@ -292,7 +291,7 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
   // 128-bit reads at ~1.5x the bytes per clock of 32-bit ones, and 16
   // z-consecutive lanes read 16 consecutive 16-byte voxels, which no bank
   // serves twice); the other modes keep channel planes [c][x][y][z]
-  constexpr bool IL = MODE == TM_PULL || (MODE == TM_SGRAD && NCC == 4);
+  constexpr bool IL = MODE == TM_PULL || (DERIV && NCC == 4);
   constexpr int DY = B::DY, DZ = B::DZ, DYZ = IL ? B::DYZ4 : B::DYZP, VOL = IL ? B::VOL4 : B::VOL;
   static_assert(NCC >= 1 && NCC <= B::MAX_NCC, "channel chunk exceeds the LDS budget");
   static_assert(!IL || NCC == 4 || NCC == 1, "pull / grad interleave 4 channels (pull may run one)");
@@ -747,14 +746,25 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
             tl_opaque(gi[0]);
             const float wyz = wi[1][0] * wi[2][0], gyz = gi[1][0] * wi[2][0], wgz = wi[1][0] * gi[2][0];
             float sw = 0.f, sd = 0.f;
+            if constexpr (IL) {   // 4 interleaved channels (zeros staged past nc): one 16-byte read a tap
+              const float4 *vq = (const float4 *)box + (bq - box);   // bq - box counts voxels here
 #pragma unroll
-            for (int kx = 0; kx < NT; ++kx) {
-              float dot = 0.f;
-              // channels >= nc of a partial chunk hold stale LDS: select, not multiply by 0
+              for (int kx = 0; kx < NT; ++kx) {
+                const float4 v = vq[kx * DYZ];
+                const float dot = v.x * tvv[0] + v.y * tvv[1] + v.z * tvv[2] + v.w * tvv[3];
+                sw += wi[0][kx] * dot;
+                sd += gi[0][kx] * dot;
+              }
+            } else {
 #pragma unroll
-              for (int c = 0; c < NCC; ++c) dot += c < nc ? bq[c * VOL + kx * DYZ] * tvv[c] : 0.f;
-              sw += wi[0][kx] * dot;
-              sd += gi[0][kx] * dot;
+              for (int kx = 0; kx < NT; ++kx) {
+                float dot = 0.f;
+                // channels >= nc of a partial chunk hold stale LDS: select, not multiply by 0
+#pragma unroll
+                for (int c = 0; c < NCC; ++c) dot += c < nc ? bq[c * VOL + kx * DYZ] * tvv[c] : 0.f;
+                sw += wi[0][kx] * dot;
+                sd += gi[0][kx] * dot;
+              }
             }
             o0 += sd * wyz;
             o1 += sw * gyz;
@@ -1295,8 +1305,7 @@ static void launch_tile(const PPArgs<float> &a, hipStream_t st) {
     if (a.C == 1)
       hipLaunchKernelGGL((tile_kernel<ORD, MODE, 1>), g, b, 0, st, a, ntx, nty, ntz, nb, fb);
     else
-      hipLaunchKernelGGL((tile_kernel<ORD, MODE, MODE == TM_SGRAD || MODE == TM_PUSH || MODE == TM_PULL ? B::MAX_NCC : 2>),
-                         g, b, 0, st, a, ntx, nty, ntz, nb, fb);
+      hipLaunchKernelGGL((tile_kernel<ORD, MODE, B::MAX_NCC>), g, b, 0, st, a, ntx, nty, ntz, nb, fb);
   }
   if (fb) {
     if constexpr (MODE == TM_PULL || MODE == TM_PUSH || MODE == TM_COUNT) {
