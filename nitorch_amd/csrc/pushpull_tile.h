@@ -838,9 +838,9 @@ __global__ __launch_bounds__((TileBox<ORD, !TileMeta<MODE>::gather>::NTH),
           }
         }
       } else {
-      // push / count: FP64 splats, one sample at a time (rolled): element 0
-      // of the per-sample state is the current sample, rotated after each
-      // (R rotations restore it for the next channel chunk)
+      // push / count at order 0: FP64 splats, one sample at a time (rolled):
+      // element 0 of the per-sample state is the current sample, rotated
+      // after each (R rotations restore it for the next channel chunk)
 #pragma unroll 1
       for (int i = 0; i < R; ++i) {
         const bool ii = inb[0];
