@@ -129,10 +129,11 @@ __global__ __launch_bounds__(BIN_NTH) void bin_sort(PPArgs<float> a, int ntx, in
     int lo[3];
     const bool ok = valid && bin_taps<ORD>(a, n, px, py, pz, xq, lo);
     const int b = ok ? (int)bin_of(lo, n, nbx, nby, nbz) : -1;
-    if (!SCATTER && MODE == TM_PULL && valid && !ok) {
+    if (!SCATTER && (MODE == TM_PULL || MODE == TM_SGRAD) && valid && !ok) {   // out of bounds: zero
       float *op = a.out + (int64_t)n * a.out_s[0] + (int64_t)px * a.out_s[2] + (int64_t)py * a.out_s[3] +
                   (int64_t)pz * a.out_s[4];
-      for (int c = 0; c < (int)a.C; ++c) op[(int64_t)c * a.out_s[1]] = 0.f;
+      for (int c = 0; c < (int)a.C; ++c)
+        for (int d = 0; d < (MODE == TM_SGRAD ? 3 : 1); ++d) op[(int64_t)c * a.out_s[1] + (int64_t)d * a.out_s[5]] = 0.f;
     }
     if (!SCATTER && ok && b < 0) idx[nsamp - 1 - atomicAdd(rest, 1)] = (int)j;
     const int rank = wave_rank(cnt, b);
@@ -327,6 +328,119 @@ __global__ __launch_bounds__(BIN_NTH) void bin_pull(PPArgs<float> a, int nbx, in
   }
 }
 
+// grid_grad: one work-group per bin, 4 channels interleaved per box voxel
+// (as bin_pull); a sample's weights and derivatives are evaluated when it
+// is processed (registers), and its taps summed separably as in the tile
+// kernel: per (z, y) tap row the x sums with weights and derivatives, then
+// the y-z factors.
+template <int ORD>
+__global__ __launch_bounds__(BIN_NTH) void bin_grad(PPArgs<float> a, int nbx, int nby, int nbz, const int *fb,
+                                                    const int *start, const int *idx, int64_t nbins) {
+  constexpr int NT = ORD + 1, E = BinBox<ORD>::E, VOL = BinBox<ORD>::VOL;
+  __shared__ __align__(16) float4 box[VOL];
+  __shared__ int fofs[3 * E];
+  __shared__ float fsg[3 * E];
+  if (fb[0] == 0) return;   // no listed tile: start[] was not written
+  const int64_t b = xcd_block(nbins);
+  const int s0 = start[b], cnt = start[b + 1] - s0;
+  if (cnt == 0) return;
+  int n, o[3];
+  bin_decode(b, nbx, nby, nbz, n, o);
+  bin_tables<ORD>(a, a.src_s, o, fofs, fsg);
+  const int C = (int)a.C, sC = (int)a.src_s[1], oC = (int)a.out_s[1], oK = (int)a.out_s[5];
+  const float *src = a.src + (int64_t)n * a.src_s[0];
+  float *out = a.out + (int64_t)n * a.out_s[0];
+  const float rY = 1.f / (float)a.tY;
+  const int gD = (int)a.grid_s[4];
+  constexpr int SPT = 2;
+  for (int i0 = 0; i0 < cnt; i0 += SPT * BIN_NTH) {
+    float q[SPT][3];
+    int off[SPT], pos[SPT];
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+      const int i = i0 + k * BIN_NTH + (int)threadIdx.x;
+      off[k] = -1;
+      pos[k] = 0;
+      q[k][0] = q[k][1] = q[k][2] = 0.f;
+      if (i >= cnt) continue;
+      const int s = idx[s0 + i];
+      const int pz = s & ((1 << BIN_ZBITS) - 1), row = s >> BIN_ZBITS;
+      const int px = tl_div(row, (int)a.tY, rY), py = row - px * (int)a.tY;
+      const float *gp = a.grid + (int64_t)n * a.grid_s[0] + (int64_t)px * a.grid_s[1] + (int64_t)py * a.grid_s[2] +
+                        (int64_t)pz * a.grid_s[3];
+      int l[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        float wt[NT], gt[NT];
+        q[k][d] = gp[d * gD];
+        l[d] = tl_taps<ORD>(q[k][d], wt, gt) - o[d];
+      }
+      off[k] = (l[0] * E + l[1]) * E + l[2];
+      pos[k] = px * (int)a.out_s[2] + py * (int)a.out_s[3] + pz * (int)a.out_s[4];
+    }
+    for (int c0 = 0; c0 < C; c0 += 4) {
+      const int nc = min(4, C - c0);
+      __syncthreads();   // tables ready / previous chunk's taps done
+      for (int qv = threadIdx.x; qv < VOL; qv += BIN_NTH) {
+        const int bx = qv / (E * E), by = (qv / E) % E, bz = qv % E;
+        const float sg = fsg[bx] * fsg[E + by] * fsg[2 * E + bz];
+        const float *sp = src + (int64_t)c0 * sC + fofs[bx] + fofs[E + by] + fofs[2 * E + bz];
+        float v[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] = (c < nc && sg != 0.f) ? sp[c * sC] * sg : 0.f;
+        box[qv] = make_float4(v[0], v[1], v[2], v[3]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < SPT; ++k) {
+        if (off[k] < 0) continue;
+        float wi[3][NT], gi[3][NT];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) tl_taps<ORD, true>(q[k][d], wi[d], gi[d]);
+        float ax[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, az[4] = {0.f, 0.f, 0.f, 0.f};
+        const float4 *bp = box + off[k];
+#pragma unroll 1
+        for (int kz = 0; kz < NT; ++kz, ++bp) {
+          const float4 *bq = bp;
+#pragma unroll 1
+          for (int ky = 0; ky < NT; ++ky, bq += E) {
+            const float wyz = wi[1][0] * wi[2][0], gyz = gi[1][0] * wi[2][0], wgz = wi[1][0] * gi[2][0];
+            float sw[4] = {0.f, 0.f, 0.f, 0.f}, sd[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kx = 0; kx < NT; ++kx) {
+              const float4 v = bq[kx * E * E];
+              const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+              for (int c = 0; c < 4; ++c) {
+                sw[c] += wi[0][kx] * vv[c];
+                sd[c] += gi[0][kx] * vv[c];
+              }
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              ax[c] += sd[c] * wyz;
+              ay[c] += sw[c] * gyz;
+              az[c] += sw[c] * wgz;
+            }
+            tl_rot(wi[1]);
+            tl_rot(gi[1]);
+          }
+          tl_rot(wi[2]);
+          tl_rot(gi[2]);
+        }
+        float *op = out + (int64_t)c0 * oC + pos[k];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (c < nc) {
+            op[c * oC] = ax[c];
+            op[c * oC + oK] = ay[c];
+            op[c * oC + 2 * oK] = az[c];
+          }
+      }
+    }
+  }
+}
+
 // Push / count: one work-group per bin; FP64 box, one channel at a time.
 template <int ORD, int MODE>
 __global__ __launch_bounds__(BIN_NTH, 4) void bin_push(PPArgs<float> a, int nbx, int nby, int nbz, const int *fb,
@@ -431,6 +545,9 @@ static bool launch_binned(const PPArgs<float> &a, int ntx, int nty, int ntz, int
                      (const int *)idx, (const int *)rest, (int)nsamp);
   if constexpr (MODE == TM_PULL)
     hipLaunchKernelGGL((bin_pull<ORD>), dim3((unsigned)nbins), dim3(BIN_NTH), 0, st, a, nbx, nby, nbz, fb,
+                       (const int *)start, (const int *)idx, nbins);
+  else if constexpr (MODE == TM_SGRAD)
+    hipLaunchKernelGGL((bin_grad<ORD>), dim3((unsigned)nbins), dim3(BIN_NTH), 0, st, a, nbx, nby, nbz, fb,
                        (const int *)start, (const int *)idx, nbins);
   else
     hipLaunchKernelGGL((bin_push<ORD, MODE>), dim3((unsigned)nbins), dim3(BIN_NTH), 0, st, a, nbx, nby, nbz, fb,

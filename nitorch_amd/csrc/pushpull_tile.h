@@ -1321,8 +1321,16 @@ static void launch_tile(const PPArgs<float> &a, hipStream_t st) {
         hipLaunchKernelGGL((tile_dyn<ORD, MODE>), dim3(gd), dim3(DYN_NTH), 0, st, a, ntx, nty, ntz, (const int *)fb);
       }
     } else {
-      const unsigned gf = (unsigned)(nb < 2048 ? nb : 2048);
-      hipLaunchKernelGGL((tile_fallback<ORD, MODE>), dim3(gf), b, 0, st, a, ntx, nty, ntz, nb, (const int *)fb);
+      // grid_grad's listed tiles take the binned pass in calls that may
+      // allocate and have BIN_MIN_SAMPLES samples; the other gradient modes,
+      // and grid_grad otherwise, run pp_sample over the listed tiles
+      bool done = false;
+      if constexpr (MODE == TM_SGRAD)
+        done = own && a.N * a.tX * a.tY * a.tZ >= BIN_MIN_SAMPLES && launch_binned<ORD, MODE>(a, ntx, nty, ntz, nb, fb, st);
+      if (!done) {
+        const unsigned gf = (unsigned)(nb < 2048 ? nb : 2048);
+        hipLaunchKernelGGL((tile_fallback<ORD, MODE>), dim3(gf), b, 0, st, a, ntx, nty, ntz, nb, (const int *)fb);
+      }
     }
     if (own) (void)hipFreeAsync(fb, st);
   } else {

@@ -1,4 +1,4 @@
-"""GPU parity of pull / push / count on strongly deformed grids, against the
+"""GPU parity of pull / push / count / grad on strongly deformed grids, against the
 FP64 oracle: rotations with zoom (affine lattices) and smooth displacements
 with a large Jacobian, and samples collapsed at random into a small
 region.  Their tiles do not fit the static LDS box.  In calls of at least
@@ -7,7 +7,7 @@ of the collapsed grid hold thousands of samples, processed in batches); in
 smaller calls each listed tile is cut into sub-tiles with boxes of their
 own.  Both routes are tested.
 
-Tolerances: pull 1e-5 norm-relative (order 0: bit-exact); push / count
+Tolerances: pull and grad 1e-5 norm-relative (pull at order 0: bit-exact); push / count
 element-wise |gpu - oracle| <= 1e-4 * oracle(|v|) (+ 1e-36, below float's
 normal range).  The affine lattices put
 float32 coordinates exactly on the out-of-bound edges; the oracle decides
@@ -84,6 +84,10 @@ def test_deformed_pull_push_count(order):
                 if order == 0:
                     assert np.array_equal(out, ref), ("pull",) + case
                 assert np.abs(out - ref).max() <= 1e-5 * np.abs(ref).max(), ("pull",) + case
+                if order > 0:
+                    out = G.grid_grad(T(x), T(g), [bound], [order], extrap).cpu().double().numpy()
+                    ref = O.grid_grad(x.astype(np.float64), g64, bound, order, extrap, oob_f32=True)
+                    assert np.abs(out - ref).max() <= 1e-5 * np.abs(ref).max(), ("grad",) + case
                 y = rng.standard_normal((1, 3) + shape).astype(np.float32)
                 y64 = y.astype(np.float64)
                 out = G.grid_push(T(y), T(g), list(shape), [bound], [order], extrap).cpu().double().numpy()
@@ -98,7 +102,7 @@ def test_deformed_pull_push_count(order):
 @pytest.mark.parametrize("order", [3])
 def test_deformed_binned_pass(order):
     """The same checks at 100 x 104 x 104 (>= 2^20 samples: the binned second
-    pass)."""
+    pass, grid_grad's included)."""
     from nitorch_amd import grid as G
     rng = np.random.default_rng(1200 + order)
     shape = (100, 104, 104)
@@ -114,6 +118,9 @@ def test_deformed_binned_pass(order):
             out = G.grid_pull(T(x), T(g), [bound], [order], extrap).cpu().double().numpy()
             ref = O.grid_pull(x.astype(np.float64), g64, bound, order, extrap, oob_f32=True)
             assert np.abs(out - ref).max() <= 1e-5 * np.abs(ref).max(), ("pull",) + case
+            out = G.grid_grad(T(x), T(g), [bound], [order], extrap).cpu().double().numpy()
+            ref = O.grid_grad(x.astype(np.float64), g64, bound, order, extrap, oob_f32=True)
+            assert np.abs(out - ref).max() <= 1e-5 * np.abs(ref).max(), ("grad",) + case
             y = rng.standard_normal((1, 2) + shape).astype(np.float32)
             y64 = y.astype(np.float64)
             out = G.grid_push(T(y), T(g), list(shape), [bound], [order], extrap).cpu().double().numpy()
