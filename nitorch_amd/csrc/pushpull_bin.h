@@ -1,5 +1,5 @@
 // Binned second pass for the tiles that tile_kernel listed (pull / push /
-// count of strongly deformed grids).
+// count / grid_grad of strongly deformed grids).
 //
 // A tile of samples that a rotation, a zoom or a displacement with a large
 // Jacobian spreads over a wide region of the source has a bounding box far
@@ -7,13 +7,14 @@
 // by the bin of their first tap instead: BIN_S^3 first-tap positions per
 // bin, in source space, so that a bin's samples touch exactly a
 // (BIN_S + ORD)^3 box of voxels whatever the deformation.  One work-group
-// per bin then stages that box (pull: 4 channels interleaved per voxel) or
-// accumulates into it (push / count: FP64, as tile_kernel), and flushes it
-// folded and signed with one FP32 global atomic per non-zero element.
+// per bin then stages that box (pull, grid_grad: 4 channels interleaved per
+// voxel) or accumulates into it (push / count: FP64, as tile_kernel) and
+// flushes it folded and signed with one FP32 global atomic per non-zero
+// element.
 //
 // Counting sort in three launches over the listed samples: count per bin,
-// exclusive scan of the counts, scatter of the sample positions.  A pull
-// writes the zeros of its out-of-bound samples in the count pass; samples
+// exclusive scan of the counts, scatter of the sample positions.  A pull or
+// grid_grad writes the zeros of its out-of-bound samples in the count pass; samples
 // whose first tap lies outside the binned range (far out-of-range
 // coordinates with extrapolation) are listed there and run the direct
 // per-sample routine pp_sample in a kernel of their own (its registers

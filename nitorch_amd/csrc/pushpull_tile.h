@@ -25,15 +25,15 @@
 // The box has fixed dimensions per order (every tap offset is then an
 // immediate of the LDS instruction); channels are staged NCC at a time.
 // A tile whose box does not fit (scattered or far out-of-range
-// coordinates with extrapolation) is listed for a second pass.  Pull, push
-// and count: in calls of at least BIN_MIN_SAMPLES samples the listed tiles'
-// samples are sorted by first-tap bin and every bin runs with a box of its
-// own (pushpull_bin.h); in smaller calls, and in the entry point that
-// allocates nothing, tile_dyn cuts each listed tile into sub-tiles with boxes
-// sized to their own spread and runs the direct per-sample routine pp_sample
-// for a sub-tile that still does not fit.  tile_fallback runs pp_sample for
-// the gradient modes.  The result never depends on the data's smoothness,
-// only the speed does.
+// coordinates with extrapolation) is listed for a second pass.  Pull, push,
+// count and grid_grad: in calls of at least BIN_MIN_SAMPLES samples the
+// listed tiles' samples are sorted by first-tap bin and every bin runs with
+// a box of its own (pushpull_bin.h).  In smaller calls of pull, push and
+// count, and in the entry point that allocates nothing, tile_dyn cuts each
+// listed tile into sub-tiles with boxes sized to their own spread and runs
+// the direct per-sample routine pp_sample for a sub-tile that still does
+// not fit; tile_fallback runs pp_sample for the other gradient cases.  The
+// result never depends on the data's smoothness, only the speed does.
 //
 // Weights, the out-of-bound rule and the splat order of each sample are
 // those of pp_sample (nitorch/_C/src/impl/pushpull_common.cpp: OOB :926-974,
